@@ -1,0 +1,237 @@
+"""Known-answer checks of the oracle's building blocks.
+
+These restate the reference's D unittest blocks (cited per test) and add
+checks that the mathematics fixes (closed forms, invariants, high-precision
+references), each chosen so that a plausible slip (a wrong sign, a dropped
+term, a transposed index, a sequential instead of pairwise sum) fails.
+"""
+import math
+import random
+
+import numpy as np
+import pytest
+
+
+def test_connectivity_tables(oracle_mod):
+    """lb/connectivity.d:432-480: bounce symmetry, weights sum to one,
+    c_s^2 = 1/3, plus the isotropy moments sum_i w_i c_ia c_ib = c_s^2 delta."""
+    for d, q in ((3, 19), (2, 9), (1, 3)):
+        vel, bounce, w, css = oracle_mod.conn(d, q)
+        assert css == 1.0 / 3.0
+        assert abs(w.sum() - 1.0) < 1e-15
+        for i in range(q):
+            assert np.all(vel[bounce[i]] == -vel[i])
+            assert bounce[bounce[i]] == i
+        m2 = np.einsum("i,ia,ib->ab", w, vel, vel)
+        assert np.allclose(m2, css * np.eye(d), atol=1e-15)
+        assert np.allclose(np.einsum("i,ia->a", w, vel), 0.0, atol=1e-16)
+    vel, _, _, _ = oracle_mod.conn(3, 19)
+    # D3Q19 order fixed by connectivity.d:401-418
+    assert vel[7].tolist() == [1, 1, 0] and vel[12].tolist() == [1, 0, -1]
+    assert vel[13].tolist() == [1, -1, 0] and vel[18].tolist() == [-1, 0, -1]
+
+
+def test_density_known_values(oracle_mod):
+    """lb/density.d:50-61."""
+    pop = np.zeros(19)
+    pop[7] = 0.1
+    assert oracle_mod.pop_density(pop) == 0.1
+    pop = np.zeros(19)
+    pop[[0, 2, 7, 10, 14]] = 0.1
+    assert oracle_mod.pop_density(pop) == 0.5
+
+
+def test_density_is_phobos_pairwise(oracle_mod):
+    """density() is std.algorithm.sum: pairwise by recursive halving.
+    The explicit tree for Q=19 must match bit for bit, and inputs exist on
+    which a sequential sum differs."""
+    rng = np.random.default_rng(5)
+
+    def tree19(p):
+        return 0.0 + ((((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + (p[7] + p[8]))))
+                      + (((p[9] + p[10]) + (p[11] + (p[12] + p[13])))
+                         + ((p[14] + p[15]) + (p[16] + (p[17] + p[18])))))
+    differs = 0
+    for _ in range(2000):
+        p = rng.uniform(0.0, 0.2, 19) * rng.choice([1e-3, 1.0, 1e3], 19)
+        s = oracle_mod.pop_density(p)
+        assert s == tree19(p)
+        seq = 0.0
+        for v in p:
+            seq += v
+        differs += seq != s
+    assert differs > 100
+
+
+def test_velocity_known_values(oracle_mod):
+    """lb/velocity.d:59-74 through the SecondOrder equilibrium at dv = 0:
+    the raw velocity of a population is recovered from its equilibrium."""
+    pop = np.zeros(19)
+    pop[7] = 0.1
+    # c_7 = (1,1,0): raw velocity (1,1,0)
+    vel, _, _, _ = oracle_mod.conn(3, 19)
+    assert (pop @ vel / oracle_mod.pop_density(pop)).tolist() == [1.0, 1.0, 0.0]
+    pop = np.zeros(19)
+    pop[[0, 2, 7, 10, 14]] = 0.1
+    u = pop @ vel / oracle_mod.pop_density(pop)
+    assert np.allclose(u, [-0.2, -0.2, 0.2], atol=1e-16)
+
+
+@pytest.mark.parametrize("form", [0, 1, 2])
+@pytest.mark.parametrize("dq", [(3, 19), (2, 9)])
+def test_eqdist_conservation(oracle_mod, form, dq):
+    """lb/eqdist.d:317-360: mass conservation for all forms; the first moment
+    of n_eq equals rho (u + dv) except for BDist2, whose first moment is
+    rho dv (it ignores the population's own velocity)."""
+    d, q = dq
+    vel, _, w, _ = oracle_mod.conn(d, q)
+    rng = np.random.default_rng(1)
+    for _ in range(50):
+        pop = rng.uniform(0.0, 1.0, q)
+        dv = rng.uniform(-1e-2, 1e-2, d)
+        rho = oracle_mod.pop_density(pop)
+        eq = oracle_mod.eq_dist(d, q, form, pop, dv, rho)
+        assert abs(eq.sum() - rho) < 1e-13 * rho
+        u = pop @ vel / rho
+        if form == 2 and d == 3:
+            assert np.allclose(eq @ vel, rho * dv, atol=1e-14)
+        elif form == 2:
+            # the D2Q9 BDist2 constants leave an O(|dv|^3) first-moment term
+            assert np.allclose(eq @ vel, rho * dv, atol=2 * rho * np.max(np.abs(dv)) ** 3)
+        else:
+            assert np.allclose(eq @ vel, rho * (u + dv), atol=1e-14)
+    # unity equilibrium (eqDistUnity): the weights for SecondOrder/ThirdOrder
+    pop0 = np.zeros(q)
+    pop0[0] = 1.0
+    eq = oracle_mod.eq_dist(d, q, form, pop0, np.zeros(d), 1.0)
+    if form != 2:
+        assert np.array_equal(eq, w)
+    elif (d, q) == (2, 9):  # BDist2 D2Q9 constants 1/2, 1/12, 1/24 (eqdist.d:185-235)
+        assert eq.tolist() == [0.5] + [1.0 / 12] * 4 + [1.0 / 24] * 4
+    else:
+        assert np.array_equal(eq, w)
+
+
+def test_secondorder_closed_form(oracle_mod):
+    """n_eq = rho w (1 + 3 c.u + 9/2 (c.u)^2 - 3/2 u^2) for D3Q19 (c_s^2 = 1/3)."""
+    vel, _, w, _ = oracle_mod.conn(3, 19)
+    rng = np.random.default_rng(2)
+    pop = rng.uniform(0.01, 0.1, 19)
+    dv = np.array([1e-3, -2e-3, 5e-4])
+    rho = oracle_mod.pop_density(pop)
+    u = pop @ vel / rho + dv
+    cu = vel @ u
+    ref = rho * w * (1 + 3 * cu + 4.5 * cu ** 2 - 1.5 * (u @ u))
+    assert np.allclose(oracle_mod.eq_dist(3, 19, 0, pop, dv, rho), ref, rtol=1e-14, atol=0)
+    ref3 = ref + rho * w * (4.5 * cu ** 3 - 4.5 * (u @ u) * cu)
+    assert np.allclose(oracle_mod.eq_dist(3, 19, 1, pop, dv, rho), ref3, rtol=1e-14, atol=0)
+
+
+def test_psi_known_values(oracle_mod):
+    """lb/force.d:370-384."""
+    assert oracle_mod.psi_value(0, 0.0) == 0.0 and oracle_mod.psi_value(1, 0.0) == 0.0
+    assert oracle_mod.psi_value(0, 0.7) == 0.7
+    assert math.isclose(oracle_mod.psi_value(1, 0.7), 0.5034146962, rel_tol=1e-9)
+    assert math.isclose(oracle_mod.psi_value(1, 1.0), 0.63212055882, rel_tol=1e-9)
+    assert math.isclose(oracle_mod.psi_value(1, 2.0), 0.86466471676, rel_tol=1e-9)
+
+
+def _mp_exp(x):
+    """exp(x) at 256 bits, rounded once to double.  mpmath's float() rounds
+    to 53 bits before scaling, which double-rounds subnormal results, so
+    those are rounded here on the 2^-1074 grid directly."""
+    import mpmath
+    with mpmath.workprec(256):
+        v = mpmath.exp(mpmath.mpf(x))
+        if v < mpmath.mpf(2) ** -1022:
+            return int(mpmath.nint(v * mpmath.mpf(2) ** 1074)) * 2.0 ** -1074
+        return float(v)
+
+
+def test_exp_correctly_rounded(oracle_mod):
+    """or_exp is the correctly rounded exp: checked against a 256-bit mpmath
+    evaluation on 20000 random arguments in the densities' range, on
+    arguments spread over the whole double range, and at special values."""
+    pytest.importorskip("mpmath")
+    rnd = random.Random(11)
+    xs = [-rnd.uniform(0.0, 3.0) for _ in range(15000)]
+    xs += [rnd.uniform(-745.0, 709.7) for _ in range(4000)]
+    xs += [rnd.uniform(-1e-6, 1e-6) for _ in range(1000)]
+    bad = [(x, oracle_mod.exp(x), _mp_exp(x)) for x in xs if oracle_mod.exp(x) != _mp_exp(x)]
+    assert not bad, bad[:5]
+    assert oracle_mod.exp(0.0) == 1.0 and oracle_mod.exp(-0.0) == 1.0
+    assert oracle_mod.exp(1.0) == _mp_exp(1.0)
+    assert oracle_mod.exp(710.0) == math.inf and oracle_mod.exp(-746.0) == 0.0
+    assert math.isnan(oracle_mod.exp(math.nan))
+    # subnormal results
+    for x in (-708.5, -720.0, -744.0):
+        assert oracle_mod.exp(x) == _mp_exp(x)
+
+
+def test_exp_quad_reference(oracle_mod):
+    """The 113-bit path used for hard cases agrees with mpmath too."""
+    pytest.importorskip("mpmath")
+    rnd = random.Random(3)
+    for _ in range(2000):
+        x = rnd.uniform(-20.0, 20.0)
+        assert oracle_mod.exp_quad(x) == _mp_exp(x)
+
+
+def test_mt19937_golden_values(oracle_mod):
+    """random.d:55-89: uniform(0.0, 2.0) golden values for seeds 0, 1, 42."""
+    gold = {0: [1.0976270048640728305144875776022672653198242187500000,
+                1.1856892330538686408658577420283108949661254882812500,
+                1.4303787302762218658358506218064576387405395507812500],
+            1: [0.8340439970684339066053780697984620928764343261718750,
+                1.9943696167306901312343825338757596909999847412109375,
+                1.4406489789114911292955412136507220566272735595703125],
+            42: [0.7490802287936861869610538633423857390880584716796875,
+                 1.5930859687722020989752991226851008832454681396484375,
+                 1.9014286235676676195538448155275546014308929443359375]}
+    for seed, vals in gold.items():
+        r = oracle_mod.Rng(seed)
+        assert [r.uniform(0.0, 2.0) for _ in range(3)] == vals
+    # raw outputs equal numpy's MT19937 with init_genrand seeding
+    r = oracle_mod.Rng(43)
+    ref = np.random.RandomState(43).randint(0, 2 ** 32, size=2000, dtype=np.uint64)
+    assert [r.next() for _ in range(2000)] == ref.tolist()
+
+
+def test_mass_and_momentum_conservation(oracle_mod):
+    """Periodic, no force, no walls: total mass is conserved to round-off and
+    total momentum stays zero-change (BDist2 and SecondOrder)."""
+    # SecondOrder conserves each component's momentum for any tau; BDist2
+    # shares one weighted velocity and conserves the total for equal tau.
+    for form, tau in (("SecondOrder", [0.8, 1.3]), ("BDist2", [0.9, 0.9])):
+        p = dict(d=3, q=19, gn=[8, 6, 5], components=2, tau=tau, eqDistForm=form,
+                 fluidInit=["EqDistPerturb"] * 2, fluidDensities=[[0.7], [0.3]],
+                 fluidPerturb=[0.05, 0.05], randomReseed=[True, True], randomSeed=[1, 2])
+        o = oracle_mod.Oracle.from_params(p)
+        m0 = [o.total_mass(f) for f in range(2)]
+        p0 = sum(o.total_momentum(f) for f in range(2))
+        o.run(20)
+        for f in range(2):
+            assert abs(o.total_mass(f) - m0[f]) < 1e-12 * m0[f]
+        p1 = sum(o.total_momentum(f) for f in range(2))
+        assert np.allclose(p1, p0, atol=1e-13)
+
+
+def test_bounce_back_wall_no_flux(oracle_mod):
+    """Full-way bounce-back: with a closed box nothing leaves the fluid, so
+    mass is conserved and the streaming of a population next to a wall
+    returns its own opposite population."""
+    p = dict(d=2, q=9, gn=[6, 5], components=1, tau=[1.0], maskInit="Box",
+             fluidInit=["EqDistPerturb"], fluidDensities=[[1.0]], fluidPerturb=[0.1],
+             randomReseed=[True], randomSeed=[7])
+    o = oracle_mod.Oracle.from_params(p)
+    m0 = o.total_mass(0)
+    pre = o.population(0).copy()
+    o.exchange_halo()
+    o.advect()
+    post = o.population(0)
+    _, bounce, _, _ = oracle_mod.conn(2, 9)
+    # site (1,1) is fluid; its neighbour in -x direction (0,1) is solid, so
+    # population q=1 (c=(1,0)) comes from its own q=2 population
+    assert post[1, 1, 1] == pre[1, 1, bounce[1]]
+    o.run(30)
+    assert abs(o.total_mass(0) - m0) < 1e-12 * m0
