@@ -1,0 +1,741 @@
+// dlbc_hip.hip -- context, data movement, reductions and the C ABI of
+// include/dlbc_hip.h.  The time-step kernels live in kernels_*.hpp.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dlbc_hip.h"
+#include "kernels_fused.hpp"
+#include "kernels_twopass.hpp"
+
+using namespace dlbc;
+
+namespace {
+
+constexpr int kMaxC = 4;
+constexpr int kMaxKernels = 8;
+
+}  // namespace
+
+struct dlbc_gpu_ctx {
+  int d = 0, q = 0, C = 0, storage = 0, device = 0;
+  int n[3] = {1, 1, 1};
+  Geo geo{};
+  hipStream_t stream = nullptr;
+  double *popA = nullptr, *popB = nullptr;  // FP64 storage
+  double *S = nullptr;                      // psi (fluid) / rho (solid)
+  uint8_t *mask = nullptr;
+  int has_solid = 0;
+  bool params_set = false, pops_uploaded[kMaxC] = {false, false, false, false};
+  bool solid_rho_dirty = true;
+  StepParams p{};
+  int schedule = DLBC_SCHED_AUTO;
+  std::string err;
+  int64_t dev_bytes = 0;
+  hipEvent_t ev[2 * kMaxKernels + 2] = {};
+  bool events_ready = false;
+  // scratch for reductions and transfers
+  double *scratch = nullptr;
+  size_t scratch_bytes = 0;
+};
+
+namespace {
+
+int fail(dlbc_gpu_ctx *c, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                                   \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return fail((ctx), DLBC_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));      \
+  } while (0)
+
+int ensure_scratch(dlbc_gpu_ctx *c, size_t bytes) {
+  if (c->scratch_bytes >= bytes) return DLBC_OK;
+  if (c->scratch) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipFree(c->scratch));
+    c->dev_bytes -= (int64_t)c->scratch_bytes;
+    c->scratch = nullptr;
+    c->scratch_bytes = 0;
+  }
+  if (hipMalloc(&c->scratch, bytes) != hipSuccess)
+    return fail(c, DLBC_ERR_NOMEM, "cannot allocate %zu bytes of device scratch", bytes);
+  c->scratch_bytes = bytes;
+  c->dev_bytes += (int64_t)bytes;
+  return DLBC_OK;
+}
+
+inline int blocks_for(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
+
+// ---- layout conversion kernels ---------------------------------------------
+// AoS staging [s][q] (physical sites s0.. s0+cnt) <-> SoA component f.
+__global__ void k_aos_to_soa(const double *__restrict__ stg, double *__restrict__ A,
+                             double *__restrict__ B, int q, int f, int64_t sp, int s0, int cnt) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= cnt) return;
+  for (int i = 0; i < q; ++i) {
+    const double v = stg[(int64_t)t * q + i];
+    A[(int64_t)(f * q + i) * sp + s0 + t] = v;
+    B[(int64_t)(f * q + i) * sp + s0 + t] = v;
+  }
+}
+
+__global__ void k_soa_to_aos(const double *__restrict__ A, double *__restrict__ stg, int q, int f,
+                             int64_t sp, int s0, int cnt) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= cnt) return;
+  for (int i = 0; i < q; ++i) stg[(int64_t)t * q + i] = A[(int64_t)(f * q + i) * sp + s0 + t];
+}
+
+template <class L>
+__global__ void k_solid_rho(const double *__restrict__ A, const uint8_t *__restrict__ mask,
+                            double *__restrict__ S, int C, Geo g) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= g.sites || !mask[s]) return;
+  for (int f = 0; f < C; ++f) {
+    double n[L::q];
+#pragma unroll
+    for (int i = 0; i < L::q; ++i) n[i] = A[(int64_t)(f * L::q + i) * g.sp + s];
+    S[f * g.sp + s] = density<L>(n);
+  }
+}
+
+// Output fields (src/dlbc/lb/io.d:56-101).  kind 0: density of f1 (all sites);
+// 1: raw velocity of f1 (lb/velocity.d:37-48, 106-117; 0 on solid);
+// 2: colour rho_f1 - rho_f2 (lb/density.d:284-295; 0 on solid).
+template <class L>
+__global__ void k_field(const double *__restrict__ A, const uint8_t *__restrict__ mask,
+                        double *__restrict__ out, Geo g, int kind, int f1, int f2, int has_solid) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= g.sites) return;
+  double n[L::q];
+#pragma unroll
+  for (int i = 0; i < L::q; ++i) n[i] = A[(int64_t)(f1 * L::q + i) * g.sp + s];
+  const bool solid = has_solid && mask[s];
+  if (kind == 0) {
+    out[s] = density<L>(n);
+  } else if (kind == 1) {
+    double v[L::d];
+#pragma unroll
+    for (int j = 0; j < L::d; ++j) v[j] = 0.0;
+    if (!solid) {
+      const double rho = density<L>(n);
+#pragma unroll
+      for (int i = 0; i < L::q; ++i)
+#pragma unroll
+        for (int j = 0; j < L::d; ++j) v[j] += n[i] * static_cast<double>(L::c[i][j]);
+#pragma unroll
+      for (int j = 0; j < L::d; ++j) v[j] /= rho;
+    }
+#pragma unroll
+    for (int j = 0; j < L::d; ++j) out[(int64_t)s * L::d + j] = v[j];
+  } else {
+    if (solid) {
+      out[s] = 0.0;
+    } else {
+      double m[L::q];
+#pragma unroll
+      for (int i = 0; i < L::q; ++i) m[i] = A[(int64_t)(f2 * L::q + i) * g.sp + s];
+      out[s] = density<L>(n) - density<L>(m);
+    }
+  }
+}
+
+// Deterministic double-double reduction over fluid sites.  Stage 1: every
+// block sums a fixed contiguous chunk (thread-sequential then tree); stage 2:
+// one block sums the block partials the same way.  what: 0 mass, 1..3 momentum.
+__device__ inline DD dd_plus(DD a, DD b) { return dd_add(a, b); }
+
+template <class L>
+__global__ void __launch_bounds__(256) k_reduce1(const double *__restrict__ A,
+                                                 const uint8_t *__restrict__ mask,
+                                                 double *__restrict__ part, Geo g, int f,
+                                                 int what, int chunk, int has_solid) {
+  __shared__ double sh[256], sl[256];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const int begin = b * chunk;
+  const int end = min(begin + chunk, g.sites);
+  DD acc{0.0, 0.0};
+  for (int s = begin + t; s < end; s += 256) {
+    if (has_solid && mask[s]) continue;
+    double n[L::q];
+#pragma unroll
+    for (int i = 0; i < L::q; ++i) n[i] = A[(int64_t)(f * L::q + i) * g.sp + s];
+    double v;
+    if (what == 0) {
+      v = density<L>(n);
+    } else {
+      v = 0.0;  // momentum (lb/momentum.d:37-52): sum n_i c_i in i order
+#pragma unroll
+      for (int i = 0; i < L::q; ++i) v += n[i] * static_cast<double>(L::c[i][what - 1]);
+    }
+    acc = dd_plus(acc, DD{v, 0.0});
+  }
+  sh[t] = acc.hi;
+  sl[t] = acc.lo;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      const DD r = dd_plus(DD{sh[t], sl[t]}, DD{sh[t + w], sl[t + w]});
+      sh[t] = r.hi;
+      sl[t] = r.lo;
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    part[2 * b] = sh[0];
+    part[2 * b + 1] = sl[0];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_reduce2(const double *__restrict__ part, int nparts,
+                                                 double *__restrict__ out) {
+  __shared__ double sh[256], sl[256];
+  const int t = threadIdx.x;
+  DD acc{0.0, 0.0};
+  for (int i = t; i < nparts; i += 256) acc = dd_plus(acc, DD{part[2 * i], part[2 * i + 1]});
+  sh[t] = acc.hi;
+  sl[t] = acc.lo;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      const DD r = dd_plus(DD{sh[t], sl[t]}, DD{sh[t + w], sl[t + w]});
+      sh[t] = r.hi;
+      sl[t] = r.lo;
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    out[0] = sh[0];
+    out[1] = sl[0];
+  }
+}
+
+__global__ void k_eval_psi(const double *__restrict__ x, double *__restrict__ y, int64_t n,
+                           int form) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) y[i] = psi_of(form, x[i]);
+}
+
+// ---- dispatch over connectivity and component count ------------------------
+template <template <class, int> class Fn, class... Args>
+int dispatch(dlbc_gpu_ctx *c, Args... args) {
+  auto go = [&](auto lat) -> int {
+    using L = decltype(lat);
+    switch (c->C) {
+      case 1: return Fn<L, 1>::run(c, args...);
+      case 2: return Fn<L, 2>::run(c, args...);
+      case 3: return Fn<L, 3>::run(c, args...);
+      case 4: return Fn<L, 4>::run(c, args...);
+    }
+    return fail(c, DLBC_ERR_UNSUPPORTED, "components %d", c->C);
+  };
+  if (c->d == 3) return go(D3Q19{});
+  if (c->d == 2) return go(D2Q9{});
+  return go(D1Q3{});
+}
+
+template <class L>
+int with_lattice(dlbc_gpu_ctx *c, int (*fn)(dlbc_gpu_ctx *, L)) {
+  return fn(c, L{});
+}
+
+// Launch wrapper that optionally brackets the launch with events.
+struct Timer {
+  dlbc_gpu_ctx *c;
+  bool on;
+  int slot = 0;
+  void before() {
+    if (on) (void)hipEventRecord(c->ev[2 * slot], c->stream);
+  }
+  void after() {
+    if (on) (void)hipEventRecord(c->ev[2 * slot + 1], c->stream);
+    ++slot;
+  }
+};
+
+template <class L, int C>
+struct TwoPassStep {
+  static int run(dlbc_gpu_ctx *c, Timer *tm, double *acc_ms) {
+    const int nb = blocks_for(c->geo.sites, 256);
+    if (c->p.sc) {
+      tm->before();
+      hipLaunchKernelGGL((k_psi<L, C>), dim3(nb), dim3(256), 0, c->stream, c->popA, c->mask,
+                         c->S, c->geo, c->p.psi_form, c->has_solid);
+      tm->after();
+    }
+    tm->before();
+    hipLaunchKernelGGL((k_collide<L, C>), dim3(nb), dim3(256), 0, c->stream, c->popA, c->popB,
+                       c->mask, c->S, c->geo, c->p);
+    tm->after();
+    (void)acc_ms;
+    std::swap(c->popA, c->popB);
+    return DLBC_OK;
+  }
+};
+
+template <class L, int C>
+struct FusedStep {
+  static int run(dlbc_gpu_ctx *c, Timer *tm, double *acc_ms) {
+    (void)acc_ms;
+    if constexpr (std::is_same<L, D3Q19>::value && C == 2) {
+      tm->before();
+      const int rc = launch_fused_d3q19(c->popA, c->popB, c->mask, c->S, c->geo, c->p, c->stream);
+      tm->after();
+      if (rc) return fail(c, DLBC_ERR_UNSUPPORTED, "fused kernel: lattice shape not supported");
+      std::swap(c->popA, c->popB);
+      return DLBC_OK;
+    } else {
+      return fail(c, DLBC_ERR_UNSUPPORTED, "fused schedule needs D3Q19 with 2 components");
+    }
+  }
+};
+
+template <class L, int C>
+struct AdvectOnly {
+  static int run(dlbc_gpu_ctx *c) {
+    hipLaunchKernelGGL((k_advect<L, C>), dim3(blocks_for(c->geo.sites, 256)), dim3(256), 0,
+                       c->stream, c->popA, c->popB, c->mask, c->geo, c->has_solid);
+    std::swap(c->popA, c->popB);
+    return DLBC_OK;
+  }
+};
+
+int effective_schedule(const dlbc_gpu_ctx *c) {
+  if (c->schedule != DLBC_SCHED_AUTO) return c->schedule;
+  if (c->d == 3 && c->C == 2 && fused_d3q19_supported(c->geo)) return DLBC_SCHED_FUSED;
+  return DLBC_SCHED_TWO_PASS;
+}
+
+int prepare(dlbc_gpu_ctx *c) {
+  if (!c->params_set) return fail(c, DLBC_ERR_STATE, "dlbc_gpu_set_params not called");
+  for (int f = 0; f < c->C; ++f)
+    if (!c->pops_uploaded[f]) return fail(c, DLBC_ERR_STATE, "component %d not uploaded", f);
+  if (c->solid_rho_dirty) {
+    if (c->has_solid) {
+      const int nb = blocks_for(c->geo.sites, 256);
+      if (c->d == 3)
+        hipLaunchKernelGGL(k_solid_rho<D3Q19>, dim3(nb), dim3(256), 0, c->stream, c->popA,
+                           c->mask, c->S, c->C, c->geo);
+      else if (c->d == 2)
+        hipLaunchKernelGGL(k_solid_rho<D2Q9>, dim3(nb), dim3(256), 0, c->stream, c->popA,
+                           c->mask, c->S, c->C, c->geo);
+      else
+        hipLaunchKernelGGL(k_solid_rho<D1Q3>, dim3(nb), dim3(256), 0, c->stream, c->popA,
+                           c->mask, c->S, c->C, c->geo);
+      HIPCHK(c, hipGetLastError());
+    }
+    c->solid_rho_dirty = false;
+  }
+  return DLBC_OK;
+}
+
+int do_steps(dlbc_gpu_ctx *c, int nsteps, Timer *tm) {
+  int rc = prepare(c);
+  if (rc) return rc;
+  const int sched = effective_schedule(c);
+  for (int t = 0; t < nsteps; ++t) {
+    tm->slot = 0;
+    if (sched == DLBC_SCHED_FUSED)
+      rc = dispatch<FusedStep>(c, tm, (double *)nullptr);
+    else
+      rc = dispatch<TwoPassStep>(c, tm, (double *)nullptr);
+    if (rc) return rc;
+  }
+  HIPCHK(c, hipGetLastError());
+  return DLBC_OK;
+}
+
+// host index of physical site (x,y,z) in a halo-padded array of the lattice
+inline int64_t host_index(const dlbc_gpu_ctx *c, int halo, int x, int y, int z) {
+  const int64_t hy = c->d >= 2 ? c->n[1] + 2 * halo : 1;
+  const int64_t hz = c->d >= 3 ? c->n[2] + 2 * halo : 1;
+  const int ox = halo, oy = c->d >= 2 ? halo : 0, oz = c->d >= 3 ? halo : 0;
+  return ((int64_t)(x + ox) * hy + (y + oy)) * hz + (z + oz);
+}
+
+// Copy a per-site field (k values per site) from device `dev` (site-major)
+// into a halo-padded host array.
+int download_sitefield(dlbc_gpu_ctx *c, const double *dev, int k, double *host, int halo) {
+  std::vector<double> buf((size_t)c->geo.sites * k);
+  HIPCHK(c, hipMemcpyAsync(buf.data(), dev, buf.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  size_t o = 0;
+  for (int x = 0; x < c->n[0]; ++x)
+    for (int y = 0; y < c->n[1]; ++y)
+      for (int z = 0; z < c->n[2]; ++z) {
+        std::memcpy(host + host_index(c, halo, x, y, z) * k, buf.data() + o * k,
+                    sizeof(double) * k);
+        ++o;
+      }
+  return DLBC_OK;
+}
+
+int field_launch(dlbc_gpu_ctx *c, double *out, int kind, int f1, int f2) {
+  const int nb = blocks_for(c->geo.sites, 256);
+  if (c->d == 3)
+    hipLaunchKernelGGL(k_field<D3Q19>, dim3(nb), dim3(256), 0, c->stream, c->popA, c->mask, out,
+                       c->geo, kind, f1, f2, c->has_solid);
+  else if (c->d == 2)
+    hipLaunchKernelGGL(k_field<D2Q9>, dim3(nb), dim3(256), 0, c->stream, c->popA, c->mask, out,
+                       c->geo, kind, f1, f2, c->has_solid);
+  else
+    hipLaunchKernelGGL(k_field<D1Q3>, dim3(nb), dim3(256), 0, c->stream, c->popA, c->mask, out,
+                       c->geo, kind, f1, f2, c->has_solid);
+  HIPCHK(c, hipGetLastError());
+  return DLBC_OK;
+}
+
+int reduce_launch(dlbc_gpu_ctx *c, int f, int what, double *out2) {
+  const int chunk = 1 << 16;
+  const int nparts = blocks_for(c->geo.sites, chunk);
+  int rc = ensure_scratch(c, sizeof(double) * (2 * (size_t)nparts + 2));
+  if (rc) return rc;
+  double *part = c->scratch, *res = c->scratch + 2 * nparts;
+  if (c->d == 3)
+    hipLaunchKernelGGL(k_reduce1<D3Q19>, dim3(nparts), dim3(256), 0, c->stream, c->popA, c->mask,
+                       part, c->geo, f, what, chunk, c->has_solid);
+  else if (c->d == 2)
+    hipLaunchKernelGGL(k_reduce1<D2Q9>, dim3(nparts), dim3(256), 0, c->stream, c->popA, c->mask,
+                       part, c->geo, f, what, chunk, c->has_solid);
+  else
+    hipLaunchKernelGGL(k_reduce1<D1Q3>, dim3(nparts), dim3(256), 0, c->stream, c->popA, c->mask,
+                       part, c->geo, f, what, chunk, c->has_solid);
+  hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(256), 0, c->stream, part, nparts, res);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(out2, res, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return DLBC_OK;
+}
+
+}  // namespace
+
+// ============================================================================
+// C ABI
+// ============================================================================
+extern "C" {
+
+int dlbc_gpu_create(dlbc_gpu_ctx **out, int d, int q, int components, const int64_t *n,
+                    int storage, int device) {
+  if (!out || !n) return DLBC_ERR_ARG;
+  *out = nullptr;
+  if (!((d == 3 && q == 19) || (d == 2 && q == 9) || (d == 1 && q == 3)))
+    return DLBC_ERR_UNSUPPORTED;
+  if (components < 1 || components > kMaxC) return DLBC_ERR_UNSUPPORTED;
+  if (storage != DLBC_STORAGE_F64) return DLBC_ERR_UNSUPPORTED;
+  int64_t sites = 1;
+  for (int i = 0; i < d; ++i) {
+    if (n[i] < 2 || n[i] > (1 << 20)) return DLBC_ERR_ARG;
+    sites *= n[i];
+  }
+  if (sites > (int64_t)1 << 30) return DLBC_ERR_UNSUPPORTED;
+  auto *c = new dlbc_gpu_ctx();
+  c->d = d;
+  c->q = q;
+  c->C = components;
+  c->storage = storage;
+  c->device = device;
+  for (int i = 0; i < d; ++i) c->n[i] = (int)n[i];
+  c->geo.nx = c->n[0];
+  c->geo.ny = c->n[1];
+  c->geo.nz = c->n[2];
+  c->geo.sites = (int)sites;
+  c->geo.sp = (sites + 63) / 64 * 64;
+  auto bail = [&](int code) {
+    dlbc_gpu_destroy(c);
+    return code;
+  };
+  if (hipSetDevice(device) != hipSuccess) return bail(DLBC_ERR_HIP);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(DLBC_ERR_HIP);
+  const size_t popb = sizeof(double) * (size_t)components * q * c->geo.sp;
+  if (hipMalloc(&c->popA, popb) != hipSuccess) return bail(DLBC_ERR_NOMEM);
+  if (hipMalloc(&c->popB, popb) != hipSuccess) return bail(DLBC_ERR_NOMEM);
+  if (hipMalloc(&c->S, sizeof(double) * components * c->geo.sp) != hipSuccess)
+    return bail(DLBC_ERR_NOMEM);
+  if (hipMalloc(&c->mask, c->geo.sp) != hipSuccess) return bail(DLBC_ERR_NOMEM);
+  c->dev_bytes = 2 * popb + sizeof(double) * components * c->geo.sp + c->geo.sp;
+  if (hipMemsetAsync(c->mask, 0, c->geo.sp, c->stream) != hipSuccess) return bail(DLBC_ERR_HIP);
+  if (hipMemsetAsync(c->S, 0, sizeof(double) * components * c->geo.sp, c->stream) != hipSuccess)
+    return bail(DLBC_ERR_HIP);
+  for (auto &e : c->ev)
+    if (hipEventCreate(&e) != hipSuccess) return bail(DLBC_ERR_HIP);
+  c->events_ready = true;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(DLBC_ERR_HIP);
+  *out = c;
+  return DLBC_OK;
+}
+
+int dlbc_gpu_set_params(dlbc_gpu_ctx *c, const double *tau, const double *gcc, const double *gwc,
+                        const double *global_acc, const double *body_force, int eq_dist_form,
+                        int psi_form, int enable_shan_chen, const double *rho0) {
+  if (!c) return DLBC_ERR_ARG;
+  if (!tau) return fail(c, DLBC_ERR_ARG, "tau is required");
+  if (eq_dist_form < 0 || eq_dist_form > 2 || psi_form < 0 || psi_form > 1)
+    return fail(c, DLBC_ERR_ARG, "bad eq_dist_form/psi_form");
+  if (enable_shan_chen && !gcc) return fail(c, DLBC_ERR_ARG, "gcc required with Shan-Chen");
+  (void)rho0;
+  StepParams p{};
+  for (int f = 0; f < c->C; ++f) {
+    p.tau[f] = tau[f];
+    p.omega[f] = 1.0 / tau[f];  // collision.d:199
+    p.one_minus_omega[f] = 1.0 - p.omega[f];
+    p.gwc[f] = gwc ? gwc[f] : 0.0;
+    for (int g = 0; g < c->C; ++g) p.gcc[f][g] = gcc ? gcc[f * c->C + g] : 0.0;
+    for (int k = 0; k < c->d; ++k) {
+      p.body[f][k] = body_force ? body_force[f * c->d + k] : 0.0;
+      if (p.body[f][k] != 0.0) p.has_body = 1;
+    }
+  }
+  for (int k = 0; k < c->d; ++k) p.gacc[k] = global_acc ? global_acc[k] : 0.0;
+  p.eq_form = eq_dist_form;
+  p.psi_form = psi_form;
+  p.sc = enable_shan_chen ? 1 : 0;
+  p.has_solid = c->has_solid;
+  c->p = p;
+  c->params_set = true;
+  return DLBC_OK;
+}
+
+int dlbc_gpu_upload_mask(dlbc_gpu_ctx *c, const int32_t *host, int halo) {
+  if (!c || !host || halo < 0) return DLBC_ERR_ARG;
+  std::vector<uint8_t> m((size_t)c->geo.sites);
+  int solid = 0;
+  size_t o = 0;
+  for (int x = 0; x < c->n[0]; ++x)
+    for (int y = 0; y < c->n[1]; ++y)
+      for (int z = 0; z < c->n[2]; ++z) {
+        const int32_t v = host[host_index(c, halo, x, y, z)];
+        if (v != 0 && v != 1) return fail(c, DLBC_ERR_ARG, "mask value %d is not 0 or 1", v);
+        m[o++] = (uint8_t)v;
+        solid |= v;
+      }
+  HIPCHK(c, hipMemcpyAsync(c->mask, m.data(), m.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->has_solid = solid;
+  c->p.has_solid = solid;
+  c->solid_rho_dirty = true;
+  return DLBC_OK;
+}
+
+int dlbc_gpu_upload_populations(dlbc_gpu_ctx *c, int f, const double *host, int halo) {
+  if (!c || !host || halo < 0 || f < 0 || f >= c->C) return DLBC_ERR_ARG;
+  const int q = c->q;
+  const int plane = c->n[1] * c->n[2];
+  const int xchunk = std::max(1, std::min(c->n[0], (int)((64u << 20) / (sizeof(double) * q * plane))));
+  int rc = ensure_scratch(c, sizeof(double) * (size_t)q * plane * xchunk);
+  if (rc) return rc;
+  std::vector<double> stg((size_t)q * plane * xchunk);
+  for (int x0 = 0; x0 < c->n[0]; x0 += xchunk) {
+    const int nxc = std::min(xchunk, c->n[0] - x0);
+    size_t o = 0;
+    for (int x = x0; x < x0 + nxc; ++x)
+      for (int y = 0; y < c->n[1]; ++y) {
+        // a z-row of physical sites is contiguous in the host array
+        std::memcpy(stg.data() + o * q, host + host_index(c, halo, x, y, 0) * q,
+                    sizeof(double) * q * c->n[2]);
+        o += c->n[2];
+      }
+    HIPCHK(c, hipMemcpyAsync(c->scratch, stg.data(), sizeof(double) * o * q,
+                             hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_aos_to_soa, dim3(blocks_for((int64_t)o, 256)), dim3(256), 0, c->stream,
+                       c->scratch, c->popA, c->popB, q, f, c->geo.sp, x0 * plane, (int)o);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  c->pops_uploaded[f] = true;
+  c->solid_rho_dirty = true;
+  return DLBC_OK;
+}
+
+int dlbc_gpu_download_populations(dlbc_gpu_ctx *c, int f, double *host, int halo) {
+  if (!c || !host || halo < 0 || f < 0 || f >= c->C) return DLBC_ERR_ARG;
+  const int q = c->q;
+  const int plane = c->n[1] * c->n[2];
+  const int xchunk = std::max(1, std::min(c->n[0], (int)((64u << 20) / (sizeof(double) * q * plane))));
+  int rc = ensure_scratch(c, sizeof(double) * (size_t)q * plane * xchunk);
+  if (rc) return rc;
+  std::vector<double> stg((size_t)q * plane * xchunk);
+  for (int x0 = 0; x0 < c->n[0]; x0 += xchunk) {
+    const int nxc = std::min(xchunk, c->n[0] - x0);
+    const int cnt = nxc * plane;
+    hipLaunchKernelGGL(k_soa_to_aos, dim3(blocks_for(cnt, 256)), dim3(256), 0, c->stream,
+                       c->popA, c->scratch, q, f, c->geo.sp, x0 * plane, cnt);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(stg.data(), c->scratch, sizeof(double) * cnt * q,
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    size_t o = 0;
+    for (int x = x0; x < x0 + nxc; ++x)
+      for (int y = 0; y < c->n[1]; ++y) {
+        std::memcpy(host + host_index(c, halo, x, y, 0) * q, stg.data() + o * q,
+                    sizeof(double) * q * c->n[2]);
+        o += c->n[2];
+      }
+  }
+  return DLBC_OK;
+}
+
+int dlbc_gpu_step(dlbc_gpu_ctx *c, int nsteps) {
+  if (!c || nsteps < 0) return DLBC_ERR_ARG;
+  Timer tm{c, false};
+  return do_steps(c, nsteps, &tm);
+}
+
+int dlbc_gpu_advect(dlbc_gpu_ctx *c) {
+  if (!c) return DLBC_ERR_ARG;
+  int rc = prepare(c);
+  if (rc) return rc;
+  rc = dispatch<AdvectOnly>(c);
+  if (rc) return rc;
+  HIPCHK(c, hipGetLastError());
+  return DLBC_OK;
+}
+
+int dlbc_gpu_step_timed(dlbc_gpu_ctx *c, int nsteps, double *elapsed_ms, double *kernel_ms,
+                        int *nkernels) {
+  if (!c || nsteps < 1) return DLBC_ERR_ARG;
+  int rc = prepare(c);
+  if (rc) return rc;
+  double acc[kMaxKernels] = {0};
+  int nk = 0;
+  hipEvent_t t0 = c->ev[2 * kMaxKernels], t1 = c->ev[2 * kMaxKernels + 1];
+  HIPCHK(c, hipEventRecord(t0, c->stream));
+  for (int t = 0; t < nsteps; ++t) {
+    Timer tm{c, true};
+    rc = do_steps(c, 1, &tm);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    nk = tm.slot;
+    for (int k = 0; k < nk; ++k) {
+      float ms = 0;
+      HIPCHK(c, hipEventElapsedTime(&ms, c->ev[2 * k], c->ev[2 * k + 1]));
+      acc[k] += ms;
+    }
+  }
+  HIPCHK(c, hipEventRecord(t1, c->stream));
+  HIPCHK(c, hipEventSynchronize(t1));
+  float tot = 0;
+  HIPCHK(c, hipEventElapsedTime(&tot, t0, t1));
+  if (elapsed_ms) *elapsed_ms = tot;
+  if (kernel_ms)
+    for (int k = 0; k < nk; ++k) kernel_ms[k] = acc[k] / nsteps;
+  if (nkernels) *nkernels = nk;
+  return DLBC_OK;
+}
+
+const char *dlbc_gpu_kernel_name(const dlbc_gpu_ctx *c, int i) {
+  if (!c) return "";
+  const int sched = effective_schedule(c);
+  if (sched == DLBC_SCHED_FUSED) return i == 0 ? "fused_d3q19" : "";
+  if (c->p.sc) return i == 0 ? "psi" : (i == 1 ? "stream_force_collide" : "");
+  return i == 0 ? "stream_collide" : "";
+}
+
+int dlbc_gpu_synchronize(dlbc_gpu_ctx *c) {
+  if (!c) return DLBC_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return DLBC_OK;
+}
+
+int dlbc_gpu_set_schedule(dlbc_gpu_ctx *c, int schedule) {
+  if (!c || schedule < 0 || schedule > 2) return DLBC_ERR_ARG;
+  if (schedule == DLBC_SCHED_FUSED && !(c->d == 3 && c->C == 2 && fused_d3q19_supported(c->geo)))
+    return fail(c, DLBC_ERR_UNSUPPORTED, "fused schedule needs D3Q19, 2 components and nz %% 64 == 0");
+  c->schedule = schedule;
+  return DLBC_OK;
+}
+
+int dlbc_gpu_get_schedule(const dlbc_gpu_ctx *c) { return c ? effective_schedule(c) : -1; }
+
+int dlbc_gpu_density(dlbc_gpu_ctx *c, int f, double *host, int halo) {
+  if (!c || !host || halo < 0 || f < 0 || f >= c->C) return DLBC_ERR_ARG;
+  int rc = ensure_scratch(c, sizeof(double) * c->geo.sites);
+  if (!rc) rc = field_launch(c, c->scratch, 0, f, f);
+  if (!rc) rc = download_sitefield(c, c->scratch, 1, host, halo);
+  return rc;
+}
+
+int dlbc_gpu_velocity(dlbc_gpu_ctx *c, int f, double *host, int halo) {
+  if (!c || !host || halo < 0 || f < 0 || f >= c->C) return DLBC_ERR_ARG;
+  int rc = ensure_scratch(c, sizeof(double) * c->geo.sites * c->d);
+  if (!rc) rc = field_launch(c, c->scratch, 1, f, f);
+  if (!rc) rc = download_sitefield(c, c->scratch, c->d, host, halo);
+  return rc;
+}
+
+int dlbc_gpu_colour(dlbc_gpu_ctx *c, int f1, int f2, double *host, int halo) {
+  if (!c || !host || halo < 0 || f1 < 0 || f1 >= c->C || f2 < 0 || f2 >= c->C)
+    return DLBC_ERR_ARG;
+  int rc = ensure_scratch(c, sizeof(double) * c->geo.sites);
+  if (!rc) rc = field_launch(c, c->scratch, 2, f1, f2);
+  if (!rc) rc = download_sitefield(c, c->scratch, 1, host, halo);
+  return rc;
+}
+
+int dlbc_gpu_total_mass(dlbc_gpu_ctx *c, int f, double *out) {
+  if (!c || !out || f < 0 || f >= c->C) return DLBC_ERR_ARG;
+  return reduce_launch(c, f, 0, out);
+}
+
+int dlbc_gpu_total_momentum(dlbc_gpu_ctx *c, int f, double *out) {
+  if (!c || !out || f < 0 || f >= c->C) return DLBC_ERR_ARG;
+  for (int j = 0; j < c->d; ++j) {
+    double r[2];
+    int rc = reduce_launch(c, f, j + 1, r);
+    if (rc) return rc;
+    out[j] = r[0];
+    out[c->d + j] = r[1];
+  }
+  return DLBC_OK;
+}
+
+int dlbc_gpu_eval_psi(dlbc_gpu_ctx *c, int psi_form, const double *x, double *y, int64_t n) {
+  if (!c || !x || !y || n < 0 || psi_form < 0 || psi_form > 1) return DLBC_ERR_ARG;
+  if (n == 0) return DLBC_OK;
+  int rc = ensure_scratch(c, sizeof(double) * 2 * n);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->scratch, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_eval_psi, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, c->scratch,
+                     c->scratch + n, n, psi_form);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(y, c->scratch + n, sizeof(double) * n, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return DLBC_OK;
+}
+
+const char *dlbc_gpu_last_error(const dlbc_gpu_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int64_t dlbc_gpu_device_bytes(const dlbc_gpu_ctx *c) { return c ? c->dev_bytes : 0; }
+
+void dlbc_gpu_destroy(dlbc_gpu_ctx *c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->popA) (void)hipFree(c->popA);
+  if (c->popB) (void)hipFree(c->popB);
+  if (c->S) (void)hipFree(c->S);
+  if (c->mask) (void)hipFree(c->mask);
+  if (c->scratch) (void)hipFree(c->scratch);
+  for (auto &e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+}  // extern "C"

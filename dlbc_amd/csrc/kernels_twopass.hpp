@@ -1,0 +1,235 @@
+// kernels_twopass.hpp -- the general two-pass schedule of one time step.
+//
+// Device layout (all kernels): structure of arrays without halo,
+//   pop[(f * Q + i) * sp + s],   s = (x * ny + y) * nz + z,
+// sp = the site count padded to a multiple of 64.  The periodic halo exchange
+// of the reference (src/dlbc/fields/parallel.d:40-153 on one rank) becomes a
+// modulo wrap of the neighbour index; this is exact because, on one rank,
+// every halo value the reference reads is the periodic image of a physical
+// site (for lattice lengths >= 2).
+//
+// Side buffer S[f * sp + s]: psi_f at fluid sites (written every step by the
+// psi pass); rho_f at solid sites (written once, when populations or mask are
+// uploaded; solid populations never change).  The Shan-Chen stencil reads
+// psi only at fluid neighbours and the wall term reads rho only at solid
+// neighbours (src/dlbc/lb/force.d:273,299), so one buffer serves both.
+#pragma once
+
+#include "lattice.hpp"
+
+namespace dlbc {
+
+struct Geo {
+  int nx, ny, nz;
+  int sites;
+  int64_t sp;
+};
+
+struct StepParams {
+  double tau[4], omega[4], one_minus_omega[4];
+  double gcc[4][4];
+  double gwc[4];
+  double gacc[3];
+  double body[4][3];
+  int eq_form, psi_form, sc, has_solid, has_body;
+};
+
+__device__ inline void coords(int s, const Geo &g, int &x, int &y, int &z) {
+  const int yz = g.ny * g.nz;
+  x = s / yz;
+  const int r = s - x * yz;
+  y = r / g.nz;
+  z = r - y * g.nz;
+}
+
+__device__ inline int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+__device__ inline int site_of(int x, int y, int z, const Geo &g) { return (x * g.ny + y) * g.nz + z; }
+
+// Pull streaming of one site (src/dlbc/lb/advection.d:61-89): for a fluid
+// site, n'_i(x) = solid(x - c_i) ? n_bounce(i)(x) : n_i(x - c_i).
+template <class L>
+__device__ inline void pull_sources(int x, int y, int z, const Geo &g, const uint8_t *mask,
+                                    int has_solid, int s, int *src, int *comp) {
+#pragma unroll
+  for (int i = 0; i < L::q; ++i) {
+    const int nb = site_of(wrap(x - L::c[i][0], g.nx), wrap(y - L::c[i][1], g.ny),
+                           wrap(z - L::c[i][2], g.nz), g);
+    if (has_solid && mask[nb]) {
+      src[i] = s;
+      comp[i] = L::bounce[i];
+    } else {
+      src[i] = nb;
+      comp[i] = i;
+    }
+  }
+}
+
+// advectFields alone (src/dlbc/lb/advection.d:35-89): A -> B for fluid sites;
+// solid sites hold the same populations in both buffers.
+template <class L, int C>
+__global__ void __launch_bounds__(256) k_advect(const double *__restrict__ A,
+                                                double *__restrict__ B,
+                                                const uint8_t *__restrict__ mask, Geo g,
+                                                int has_solid) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= g.sites) return;
+  if (has_solid && mask[s]) return;
+  int x, y, z;
+  coords(s, g, x, y, z);
+  int src[L::q], comp[L::q];
+  pull_sources<L>(x, y, z, g, mask, has_solid, s, src, comp);
+#pragma unroll
+  for (int f = 0; f < C; ++f)
+#pragma unroll
+    for (int i = 0; i < L::q; ++i)
+      B[(int64_t)(f * L::q + i) * g.sp + s] = A[(int64_t)(f * L::q + comp[i]) * g.sp + src[i]];
+}
+
+// Pass 1: streamed density and psi at every fluid site
+// (src/dlbc/lb/density.d:344-353, src/dlbc/lb/force.d:137-147,360-367).
+template <class L, int C>
+__global__ void __launch_bounds__(256) k_psi(const double *__restrict__ A,
+                                             const uint8_t *__restrict__ mask,
+                                             double *__restrict__ S, Geo g, int psi_form,
+                                             int has_solid) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= g.sites) return;
+  if (has_solid && mask[s]) return;
+  int x, y, z;
+  coords(s, g, x, y, z);
+  int src[L::q], comp[L::q];
+  pull_sources<L>(x, y, z, g, mask, has_solid, s, src, comp);
+#pragma unroll
+  for (int f = 0; f < C; ++f) {
+    double n[L::q];
+#pragma unroll
+    for (int i = 0; i < L::q; ++i) n[i] = A[(int64_t)(f * L::q + comp[i]) * g.sp + src[i]];
+    S[f * g.sp + s] = psi_of(psi_form, density<L>(n));
+  }
+}
+
+// Pass 2: streaming, Shan-Chen force (src/dlbc/lb/force.d:238-309),
+// distributeForce (force.d:179-197), BDist2 weighted velocity
+// (src/dlbc/lb/collision.d:156-196) and BGK collision (collision.d:90-119).
+template <class L, int C>
+__global__ void __launch_bounds__(256) k_collide(const double *__restrict__ A,
+                                                 double *__restrict__ B,
+                                                 const uint8_t *__restrict__ mask,
+                                                 const double *__restrict__ S, Geo g,
+                                                 StepParams p) {
+  constexpr int Q = L::q, D = L::d;
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= g.sites) return;
+  if (p.has_solid && mask[s]) return;  // solid populations stay as they are
+  int x, y, z;
+  coords(s, g, x, y, z);
+  double n[C][Q];
+  {
+    int src[Q], comp[Q];
+    pull_sources<L>(x, y, z, g, mask, p.has_solid, s, src, comp);
+#pragma unroll
+    for (int f = 0; f < C; ++f)
+#pragma unroll
+      for (int i = 0; i < Q; ++i) n[f][i] = A[(int64_t)(f * Q + comp[i]) * g.sp + src[i]];
+  }
+  double rho[C];
+#pragma unroll
+  for (int f = 0; f < C; ++f) rho[f] = density<L>(n[f]);
+
+  double F[C][D];
+#pragma unroll
+  for (int f = 0; f < C; ++f)
+#pragma unroll
+    for (int d = 0; d < D; ++d) F[f][d] = 0.0;  // resetForce
+
+  if (p.sc) {
+    int nbr[Q];
+    bool solid_nb[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      nbr[i] = site_of(wrap(x + L::c[i][0], g.nx), wrap(y + L::c[i][1], g.ny),
+                       wrap(z + L::c[i][2], g.nz), g);
+      solid_nb[i] = p.has_solid && mask[nbr[i]];
+    }
+#pragma unroll
+    for (int f1 = 0; f1 < C; ++f1) {
+      const double psi1 = S[f1 * g.sp + s];
+#pragma unroll
+      for (int f2 = 0; f2 < C; ++f2) {
+        const double cc = p.gcc[f1][f2];
+        if (cc == 0.0) continue;
+#pragma unroll
+        for (int i = 1; i < Q; ++i) {  // Iota!(1, q-1): i = 1 .. Q-1
+          const double psi2 = solid_nb[i] ? S[f2 * g.sp + s] : S[f2 * g.sp + nbr[i]];
+          const double pf = L::w[i] * psi1 * psi2 * cc;
+#pragma unroll
+          for (int d = 0; d < D; ++d) F[f1][d] -= pf * static_cast<double>(L::c[i][d]);
+        }
+      }
+      const double wc = p.gwc[f1];
+      if (wc != 0.0 && p.has_solid) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+          if (solid_nb[i]) {
+            const double pf = L::w[i] * psi1 * S[f1 * g.sp + nbr[i]] * wc;
+#pragma unroll
+            for (int d = 0; d < D; ++d) F[f1][d] -= pf * static_cast<double>(L::c[i][d]);
+          }
+        }
+      }
+    }
+  }
+  {  // distributeForce with forceDistributed == 0
+    double total = 0.0;
+#pragma unroll
+    for (int f = 0; f < C; ++f) total += rho[f];
+#pragma unroll
+    for (int f = 0; f < C; ++f)
+#pragma unroll
+      for (int d = 0; d < D; ++d) F[f][d] += 0.0 * (rho[f] / total);
+  }
+  if (p.has_body) {  // per-component body force, applied as a pre-collision hook
+#pragma unroll
+    for (int f = 0; f < C; ++f)
+#pragma unroll
+      for (int d = 0; d < D; ++d) F[f][d] += p.body[f][d];
+  }
+  double wv[D];
+  if (p.eq_form == kBDist2) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) wv[d] = 0.0;
+#pragma unroll
+    for (int f = 0; f < C; ++f) {
+      double per[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) per[d] = 0.0;
+#pragma unroll
+      for (int i = 0; i < Q; ++i)
+#pragma unroll
+        for (int d = 0; d < D; ++d) per[d] += n[f][i] * static_cast<double>(L::c[i][d]);
+#pragma unroll
+      for (int d = 0; d < D; ++d) wv[d] += per[d] / p.tau[f];
+    }
+    double total = 0.0;
+#pragma unroll
+    for (int f = 0; f < C; ++f) total += rho[f];
+#pragma unroll
+    for (int d = 0; d < D; ++d) wv[d] /= total;
+  }
+#pragma unroll
+  for (int f = 0; f < C; ++f) {
+    double dv[D], eq[Q];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      dv[d] = p.tau[f] * (p.gacc[d] + F[f][d] / rho[f]);
+      if (p.eq_form == kBDist2) dv[d] += wv[d];
+    }
+    eq_dist<L>(p.eq_form, n[f], dv, rho[f], eq);
+#pragma unroll
+    for (int i = 0; i < Q; ++i)
+      B[(int64_t)(f * Q + i) * g.sp + s] = n[f][i] * p.one_minus_omega[f] + p.omega[f] * eq[i];
+  }
+}
+
+}  // namespace dlbc

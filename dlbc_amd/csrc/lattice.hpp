@@ -1,0 +1,209 @@
+// lattice.hpp -- connectivities and per-site physics shared by all kernels.
+//
+// Every function restates one reference routine (cited) with the same
+// operation order, so that FP64 results are bit-identical to the reference's
+// D code compiled without fused multiply-add (build with -ffp-contract=off).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "crexp.hpp"
+
+namespace dlbc {
+
+enum EqForm : int { kSecondOrder = 0, kThirdOrder = 1, kBDist2 = 2 };
+
+// Connectivities (src/dlbc/lb/connectivity.d:198-418).  A d < 3 lattice is
+// stored as a 3-D one with unit extent in the unused axes; its velocities have
+// zero components there.
+struct D3Q19 {
+  static constexpr int d = 3, q = 19;
+  // connectivity.d:401-418
+  static constexpr int c[19][3] = {
+      {0, 0, 0},  {1, 0, 0},   {-1, 0, 0}, {0, 1, 0},   {0, -1, 0}, {0, 0, 1},   {0, 0, -1},
+      {1, 1, 0},  {-1, 1, 0},  {0, 1, 1},  {0, -1, 1},  {1, 0, 1},  {1, 0, -1},  {1, -1, 0},
+      {-1, -1, 0}, {0, 1, -1}, {0, -1, -1}, {-1, 0, 1}, {-1, 0, -1}};
+  // generateBounce (connectivity.d:198-212): index of -c_i
+  static constexpr int bounce[19] = {0, 2, 1, 4, 3, 6, 5, 14, 13, 16, 15, 18, 17, 8, 7, 10, 9, 12, 11};
+  // connectivity.d:252-256
+  static constexpr double w[19] = {1.0 / 3.0,  1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0,
+                                   1.0 / 18.0, 1.0 / 18.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+                                   1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+                                   1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+};
+
+struct D2Q9 {
+  static constexpr int d = 2, q = 9;
+  // connectivity.d:344-374
+  static constexpr int c[9][3] = {{0, 0, 0},  {1, 0, 0},  {-1, 0, 0}, {0, 1, 0},  {0, -1, 0},
+                                  {1, 1, 0},  {1, -1, 0}, {-1, 1, 0}, {-1, -1, 0}};
+  static constexpr int bounce[9] = {0, 2, 1, 4, 3, 8, 7, 6, 5};
+  // connectivity.d:267-271
+  static constexpr double w[9] = {4.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0, 1.0 / 9.0,
+                                  1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+};
+
+struct D1Q3 {
+  static constexpr int d = 1, q = 3;
+  // connectivity.d:318-324
+  static constexpr int c[3][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}};
+  static constexpr int bounce[3] = {0, 2, 1};
+  // connectivity.d:285-288
+  static constexpr double w[3] = {4.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0};
+};
+
+constexpr double kCss = 1.0 / 3.0;  // c_s^2 of all three (connectivity.d:214-248)
+
+// density(): std.algorithm.sum = 0.0 + pairwise sum by recursive halving
+// (src/dlbc/lb/density.d:37-39).
+template <int B, int N, class T>
+__host__ __device__ inline T pairwise(const T *r) {
+  if constexpr (N == 1) {
+    return r[B];
+  } else if constexpr (N == 2) {
+    return r[B] + r[B + 1];
+  } else {
+    return pairwise<B, N / 2>(r) + pairwise<B + N / 2, N - N / 2>(r);
+  }
+}
+
+template <class L, class T>
+__host__ __device__ inline T density(const T *pop) {
+  return T(0) + pairwise<0, L::q>(pop);
+}
+
+// eqDist (src/dlbc/lb/eqdist.d:61-278).  pop: the population, dv: the
+// velocity shift, rho0: its density.
+template <class L>
+__device__ inline void eq_dist(int form, const double *pop, const double *dv, double rho0,
+                               double *dist) {
+  constexpr int Q = L::q, D = L::d;
+  if (form != kBDist2) {
+    double pv[D], v[D];
+    // rawVelocity (lb/velocity.d:37-48): vel[j] += n_i c_ij over i, / rho
+#pragma unroll
+    for (int j = 0; j < D; ++j) pv[j] = 0.0;
+#pragma unroll
+    for (int i = 0; i < Q; ++i)
+#pragma unroll
+      for (int j = 0; j < D; ++j) pv[j] += pop[i] * static_cast<double>(L::c[i][j]);
+#pragma unroll
+    for (int j = 0; j < D; ++j) pv[j] /= rho0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) v[j] = dv[j] + pv[j];
+    // dotProduct (range.d:57-67): sequential from 0
+    double vdotv = 0.0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) vdotv += v[j] * v[j];
+    constexpr double css = kCss;
+    const double css2 = 2.0 * css;
+    const double css2p2 = css2 * css;
+    if (form == kSecondOrder) {  // eqdist.d:82-89
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        double vc = 0.0;
+#pragma unroll
+        for (int j = 0; j < D; ++j) vc += v[j] * static_cast<double>(L::c[i][j]);
+        dist[i] = rho0 * L::w[i] * (1.0 + (vc / css) + (vc * vc / css2p2) - (vdotv / css2));
+      }
+    } else {  // ThirdOrder, eqdist.d:90-100
+      const double css6p3 = 3.0 * css2p2 * css;
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        double vc = 0.0;
+#pragma unroll
+        for (int j = 0; j < D; ++j) vc += v[j] * static_cast<double>(L::c[i][j]);
+        dist[i] = rho0 * L::w[i] *
+                  (1.0 + (vc / css) + (vc * vc / css2p2) - (vdotv / css2) +
+                   (vc * vc * vc / css6p3) - (vdotv * vc / css2p2));
+      }
+    }
+    return;
+  }
+  // BDist2
+  if constexpr (D == 3 && Q == 19) {  // eqdist.d:102-184
+    const double aN1 = 1.0 / 36.0, aN0 = 1.0 / 3.0;
+    const double haN1 = 0.5 * aN1, aN1_6 = aN1 / 6.0;
+    const double T = 1.0 / 3.0, holdit = 1.0 / T;
+    const double tux = dv[0], tuy = dv[1], tuz = dv[2];
+    const double tuxt = tux * holdit, tuyt = tuy * holdit, tuzt = tuz * holdit;
+    const double uke = 0.5 * (tux * tuxt + tuy * tuyt + tuz * tuzt);
+    const double cst = 1.0 - uke;
+    const double const1 = aN1 * cst;
+    auto eo = [&](double cdotu, double &even, double &odd) {
+      even = const1 + haN1 * cdotu * cdotu;
+      odd = const1 * cdotu + aN1_6 * cdotu * cdotu * cdotu;
+    };
+    double e, o;
+    eo(tuxt, e, o);
+    dist[1] = 2.0 * (e + o);
+    dist[2] = 2.0 * (e - o);
+    eo(tuyt, e, o);
+    dist[3] = 2.0 * (e + o);
+    dist[4] = 2.0 * (e - o);
+    eo(tuzt, e, o);
+    dist[5] = 2.0 * (e + o);
+    dist[6] = 2.0 * (e - o);
+    eo(tuxt + tuyt, e, o);
+    dist[7] = e + o;
+    dist[14] = e - o;
+    eo(tuxt - tuyt, e, o);
+    dist[13] = e + o;
+    dist[8] = e - o;
+    eo(tuxt + tuzt, e, o);
+    dist[11] = e + o;
+    dist[18] = e - o;
+    eo(tuxt - tuzt, e, o);
+    dist[12] = e + o;
+    dist[17] = e - o;
+    eo(tuyt + tuzt, e, o);
+    dist[9] = e + o;
+    dist[16] = e - o;
+    eo(tuyt - tuzt, e, o);
+    dist[15] = e + o;
+    dist[10] = e - o;
+    dist[0] = aN0 * cst;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) dist[i] *= rho0;
+  } else if constexpr (D == 2 && Q == 9) {  // eqdist.d:185-235
+    const double aN1 = 1.0 / 24.0, aN0 = 1.0 / 2.0;
+    const double haN1 = 0.5 * aN1, aN1_6 = aN1 / 6.0;
+    const double T = 1.0 / 3.0, holdit = 1.0 / T;
+    const double tux = dv[0], tuy = dv[1];
+    const double tuxt = tux * holdit, tuyt = tuy * holdit;
+    const double uke = 0.5 * (tux * tuxt + tuy * tuyt);
+    const double cst = 1.0 - uke;
+    const double const1 = aN1 * cst;
+    auto eo = [&](double cdotu, double &even, double &odd) {
+      even = const1 + haN1 * cdotu * cdotu;
+      odd = const1 * cdotu + aN1_6 * cdotu * cdotu * cdotu;
+    };
+    double e, o;
+    eo(tuxt, e, o);
+    dist[1] = 2.0 * (e + o);
+    dist[2] = 2.0 * (e - o);
+    eo(tuyt, e, o);
+    dist[3] = 2.0 * (e + o);
+    dist[4] = 2.0 * (e - o);
+    eo(tuxt + tuyt, e, o);
+    dist[5] = e + o;
+    dist[8] = e - o;
+    eo(tuxt - tuyt, e, o);
+    dist[6] = e + o;
+    dist[7] = e - o;
+    dist[0] = aN0 * cst;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) dist[i] *= rho0;
+  } else {  // eqdist.d:269-271: other connectivities return the population
+#pragma unroll
+    for (int i = 0; i < Q; ++i) dist[i] = pop[i];
+  }
+}
+
+__device__ inline double psi_of(int psi_form, double rho) {
+  return psi_form == 0 ? rho : psi_exponential(rho);  // lb/force.d:320-332
+}
+
+}  // namespace dlbc

@@ -1,0 +1,203 @@
+/*
+ * dlbc_hip.h -- C ABI of the MI355X lattice Boltzmann step library.
+ *
+ * The library advances DLBC's per-timestep lattice Boltzmann path on one
+ * AMD Instinct MI355X (gfx950): pull streaming with mask bounce-back, the
+ * Shan-Chen psi/force with its velocity shift, and BGK collision with the
+ * SecondOrder, ThirdOrder and BDist2 equilibria.  It replaces the body of
+ * the reference's time loop (src/main.d:174-200):
+ *
+ *   L.exchangeHalo();      src/dlbc/lattice.d:200-214 -> periodic wrap in-kernel
+ *   L.advectFields();      src/dlbc/lb/advection.d:35-89
+ *   L.resetForce();        src/dlbc/lb/force.d:165-170
+ *   L.addShanChenForce();  src/dlbc/lb/force.d:223-309
+ *   L.distributeForce();   src/dlbc/lb/force.d:179-197
+ *   L.prepareToCollide();  src/dlbc/lb/collision.d:41-51,156-196
+ *   L.collideFields(tau, globalAcc);  src/dlbc/lb/collision.d:61-119
+ *
+ * with dlbc_gpu_step().  Host arrays use the reference's own layout: row-major
+ * [n0+2h][n1+2h][n2+2h][k] with the component index innermost (halo h = 2 in
+ * the reference, src/dlbc/lattice.d:96-103; h = 0 for physical-only arrays).
+ *
+ * Conventions
+ *  - Return value 0 is success; a negative DLBC_ERR_* code is a failure whose
+ *    message dlbc_gpu_last_error() returns.  The library never exits.
+ *  - The caller owns every host buffer and no host pointer is kept after a
+ *    call returns.  The library owns its device buffers and one HIP stream.
+ *  - A context is used from one thread at a time.  dlbc_gpu_step() is
+ *    stream-ordered and returns before the GPU finishes; every download
+ *    synchronises.
+ */
+#ifndef DLBC_HIP_H
+#define DLBC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dlbc_gpu_ctx dlbc_gpu_ctx;
+
+enum {
+  DLBC_OK = 0,
+  DLBC_ERR_ARG = -1,         /* invalid argument */
+  DLBC_ERR_HIP = -2,         /* HIP runtime error */
+  DLBC_ERR_NOMEM = -3,       /* device or host allocation failed */
+  DLBC_ERR_UNSUPPORTED = -4, /* configuration the library does not provide */
+  DLBC_ERR_STATE = -5        /* call out of order (e.g. step before upload) */
+};
+
+/* Population storage on the device. */
+enum {
+  DLBC_STORAGE_F64 = 0,       /* plain FP64 populations                       */
+  DLBC_STORAGE_F32_SHIFT = 1, /* f_i - w_i rho0 in FP32, FP32 arithmetic      */
+  DLBC_STORAGE_F16_SHIFT = 2  /* f_i - w_i rho0 in FP16, FP32 arithmetic      */
+};
+
+/* EqDistForm (src/dlbc/lb/eqdist.d:23-36) and PsiForm (src/dlbc/lb/force.d:48-57). */
+enum { DLBC_EQ_SECOND_ORDER = 0, DLBC_EQ_THIRD_ORDER = 1, DLBC_EQ_BDIST2 = 2 };
+enum { DLBC_PSI_LINEAR = 0, DLBC_PSI_EXPONENTIAL = 1 };
+
+/* Kernel schedule selection for dlbc_gpu_set_schedule(). */
+enum {
+  DLBC_SCHED_AUTO = 0,     /* fastest available for the configuration        */
+  DLBC_SCHED_TWO_PASS = 1, /* psi pass + stream/force/collide pass (any DdQq)  */
+  DLBC_SCHED_FUSED = 2     /* single-pass D3Q19 kernel with an LDS psi ring   */
+};
+
+/* Create a context for one lattice on HIP device `device`.
+ * d, q: connectivity (3,19), (2,9) or (1,3) -- the reference's dub
+ * configurations d3q19, d2q9 and d1q3 (src/dlbc/lb/connectivity.d:21-32).
+ * components: number of fluid components C (1..4), lb.components.
+ * n: physical lattice lengths n[0..d) (lattice.gn on one rank), each >= 2.
+ * Replaces the Lattice constructor and field allocation
+ * (src/dlbc/lattice.d:166-181, src/dlbc/lb/init.d:60-73, lb/force.d:67-107). */
+int dlbc_gpu_create(dlbc_gpu_ctx **out, int d, int q, int components, const int64_t *n,
+                    int storage, int device);
+
+/* Physical parameters (src/dlbc/lb/lb.d:39, lb/force.d:26-43, lb/eqdist.d:18).
+ * tau[C]       lb.tau
+ * gcc[C*C]     lb.force.gcc, row-major (may be NULL when SC is off)
+ * gwc[C]       lb.force.gwc (may be NULL: zero)
+ * global_acc[d] lb.force.globalAcc (may be NULL: zero)
+ * body_force[C*d] per-component force density added to F_f before the
+ *              velocity shift (no reference counterpart: the reference adds
+ *              such a force from a preCollisionHooks plugin); NULL for none
+ * rho0[C]      reference densities of the shifted storages (ignored for F64;
+ *              NULL: 1.0) */
+int dlbc_gpu_set_params(dlbc_gpu_ctx *ctx, const double *tau, const double *gcc,
+                        const double *gwc, const double *global_acc,
+                        const double *body_force, int eq_dist_form, int psi_form,
+                        int enable_shan_chen, const double *rho0);
+
+/* Upload component f from a host array [n0+2h]..[q] (halo h >= 0; only the
+ * physical sites are read).  Equivalent of initFluids / readCheckpoint
+ * followed by exchangeHalo (src/main.d:136-147). */
+int dlbc_gpu_upload_populations(dlbc_gpu_ctx *ctx, int f, const double *host, int halo);
+
+/* Upload the mask [n0+2h].. int32, 0 = Mask.None (fluid), 1 = Mask.Solid
+ * (src/dlbc/lb/mask.d:84-93).  Solid sites keep their populations forever
+ * (advection copies them, collision skips them); upload the mask before the
+ * populations. */
+int dlbc_gpu_upload_mask(dlbc_gpu_ctx *ctx, const int32_t *host, int halo);
+
+/* Advance nsteps time steps: nsteps x the runTimeloop body
+ * (src/main.d:174-200).  The state afterwards is post-collision, as the
+ * reference's at the end of a step. */
+int dlbc_gpu_step(dlbc_gpu_ctx *ctx, int nsteps);
+
+/* The streaming phase alone: advectFields (src/dlbc/lb/advection.d:35-89),
+ * pull streaming with mask bounce-back, after the periodic halo exchange.
+ * For callers that run the reference's phases individually (e.g. with
+ * their own hooks between them); dlbc_gpu_step already includes it. */
+int dlbc_gpu_advect(dlbc_gpu_ctx *ctx);
+
+/* As dlbc_gpu_step, and report the elapsed GPU time of the nsteps (HIP
+ * events on the library stream) and, per kernel of the schedule, the average
+ * duration of one launch (kernel_ms[i], i < *nkernels, at most 8 entries;
+ * events bracket every launch, so this mode adds a little overhead). */
+int dlbc_gpu_step_timed(dlbc_gpu_ctx *ctx, int nsteps, double *elapsed_ms, double *kernel_ms,
+                        int *nkernels);
+/* Name of kernel slot i of the current schedule (static string). */
+const char *dlbc_gpu_kernel_name(const dlbc_gpu_ctx *ctx, int i);
+
+int dlbc_gpu_synchronize(dlbc_gpu_ctx *ctx);
+int dlbc_gpu_set_schedule(dlbc_gpu_ctx *ctx, int schedule);
+int dlbc_gpu_get_schedule(const dlbc_gpu_ctx *ctx);
+
+/* Download component f into a host array [n0+2h]..[q]: physical sites are
+ * written, halo sites are left untouched (the caller's exchangeHalo fills
+ * them).  For shifted storage the populations are w_i rho0 + stored deviation. */
+int dlbc_gpu_download_populations(dlbc_gpu_ctx *ctx, int f, double *host, int halo);
+
+/* Output fields of src/dlbc/lb/io.d:56-101 into host arrays [n0+2h].. :
+ * density (densityField, lb/density.d:89-98; all sites incl. solid),
+ * raw velocity [..][d] (rawVelocityField, lb/velocity.d:106-117; 0 on solid),
+ * colour rho_f1 - rho_f2 (colourField, lb/density.d:284-295; 0 on solid). */
+int dlbc_gpu_density(dlbc_gpu_ctx *ctx, int f, double *host, int halo);
+int dlbc_gpu_velocity(dlbc_gpu_ctx *ctx, int f, double *host, int halo);
+int dlbc_gpu_colour(dlbc_gpu_ctx *ctx, int f1, int f2, double *host, int halo);
+
+/* Reductions over fluid sites (localTotalMass, lb/density.d:136-147;
+ * localTotalMomentum, lb/momentum.d:131-147).  Summed in a fixed order in
+ * double-double (compensated) arithmetic; out[0] is the sum rounded to
+ * double, out[1] (mass) / out[d..2d) (momentum) the low-order part. */
+int dlbc_gpu_total_mass(dlbc_gpu_ctx *ctx, int f, double *out /*2*/);
+int dlbc_gpu_total_momentum(dlbc_gpu_ctx *ctx, int f, double *out /*2d*/);
+
+/* Evaluate psi (form) on n densities on the device (parity tests of the
+ * correctly rounded exp).  x and y are host arrays. */
+int dlbc_gpu_eval_psi(dlbc_gpu_ctx *ctx, int psi_form, const double *x, double *y, int64_t n);
+
+const char *dlbc_gpu_last_error(const dlbc_gpu_ctx *ctx);
+void dlbc_gpu_destroy(dlbc_gpu_ctx *ctx);
+
+/* Bytes of device memory held by the context. */
+int64_t dlbc_gpu_device_bytes(const dlbc_gpu_ctx *ctx);
+
+/* ---------------------------------------------------------------------
+ * Host-side initial conditions (no GPU needed): the reference's init shapes
+ * (src/dlbc/fields/init.d, src/dlbc/lb/init.d:263-339) written into a
+ * halo-padded host array [n0+2h]..[q] exactly as the reference fills its
+ * fields, including one random draw per stored halo-padded site in row-major
+ * order for the seeded kinds (std.random.Mt19937, uniform!"[)").
+ * --------------------------------------------------------------------- */
+
+/* FluidInit (src/dlbc/lb/init.d:139-260). */
+enum {
+  DLBC_INIT_NONE = 0, DLBC_INIT_FILE, DLBC_INIT_CONST, DLBC_INIT_EQDIST,
+  DLBC_INIT_EQDIST_PERTURB, DLBC_INIT_EQDIST_PERTURB_FRAC, DLBC_INIT_CONST_RANDOM,
+  DLBC_INIT_EQDIST_RANDOM, DLBC_INIT_EQDIST_SPHERE, DLBC_INIT_EQDIST_SPHERE_FRAC,
+  DLBC_INIT_EQDIST_CYLINDER, DLBC_INIT_EQDIST_CYLINDER_FRAC, DLBC_INIT_EQDIST_TWO_SPHERES,
+  DLBC_INIT_EQDIST_TWO_SPHERES_FRAC, DLBC_INIT_EQDIST_LAMELLAE, DLBC_INIT_EQDIST_LAMELLAE_FRAC
+};
+
+/* Opaque Mersenne Twister state (std.random.Mt19937, src/dlbc/random.d:24). */
+typedef struct dlbc_rng {
+  uint32_t state[627];
+} dlbc_rng;
+void dlbc_rng_seed(dlbc_rng *rng, uint32_t seed);
+uint32_t dlbc_rng_next(dlbc_rng *rng);
+double dlbc_rng_uniform(dlbc_rng *rng, double a, double b);
+
+/* initFluid (src/dlbc/lb/init.d:263-339) for one component into host[], a
+ * halo-padded array with halo h.  eq_dist_form selects eqDistUnity
+ * (src/dlbc/lb/eqdist.d:310-315).  rng is used by the random kinds. */
+int dlbc_init_fluid(double *host, int d, int q, const int64_t *n, int halo, int eq_dist_form,
+                    int kind, const double *dens, int ndens, double perturb, dlbc_rng *rng,
+                    double radius, const double *offset, const double *separation, int axis,
+                    double thickness, const double *widths);
+/* initMask (src/dlbc/lb/mask.d:134-172): kind 0 None, 2 Plates, 3 Tube,
+ * 4 Box, 5 Walls; dim_has_wall[2d] as lb.mask.dimensionHasWall. */
+int dlbc_init_mask(int32_t *host, int d, const int64_t *n, int halo, int kind, int axis,
+                   int wall_offset, const int *dim_has_wall, int wall_thickness);
+/* initEqDistWall (src/dlbc/fields/init.d:469-477): solid sites get
+ * [1, 0, ..., 0]. */
+int dlbc_init_eq_dist_wall(double *host, const int32_t *mask, int d, int q, const int64_t *n,
+                           int halo);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DLBC_HIP_H */

@@ -43,6 +43,7 @@ struct or_lattice {
   double global_acc[3];
   int eq_dist_form, psi_form, enable_sc;
   double timers[7];
+  int density_fresh; /* Field.isFresh of L.density (lb/density.d:344-362) */
 };
 
 /* ------------------------------------------------------------------------ */
@@ -500,31 +501,20 @@ static void exchange_generic(const or_lattice *L, void *data, size_t elem) {
       const int h = L->h, nh = L->nh[dim];
       const int src0 = dir == 0 ? nh - 2 * h : h;
       const int dst0 = dir == 0 ? 0 : nh - h;
-      size_t cnt = (size_t)h;
-      int a, b, c, len[3] = {L->nh[0], L->nh[1], L->nh[2]};
+      /* the slab [lo, lo+h) of axis dim is (outer) blocks of (h * inner)
+       * contiguous elements, inner = product of the faster axes */
+      size_t outer = 1, inner = 1, o;
+      int a;
       char *buf;
-      size_t o = 0;
-      len[dim] = h;
-      for (a = 0; a < 3; a++)
-        if (a != dim) cnt *= (size_t)len[a];
-      buf = (char *)malloc(cnt * elem);
-      for (a = 0; a < len[0]; a++)
-        for (b = 0; b < len[1]; b++)
-          for (c = 0; c < len[2]; c++) {
-            int s[3] = {a, b, c};
-            s[dim] += src0;
-            memcpy(buf + o * elem, base + idx3(L, s[0], s[1], s[2]) * elem, elem);
-            o++;
-          }
-      o = 0;
-      for (a = 0; a < len[0]; a++)
-        for (b = 0; b < len[1]; b++)
-          for (c = 0; c < len[2]; c++) {
-            int s[3] = {a, b, c};
-            s[dim] += dst0;
-            memcpy(base + idx3(L, s[0], s[1], s[2]) * elem, buf + o * elem, elem);
-            o++;
-          }
+      for (a = 0; a < dim; a++) outer *= (size_t)L->nh[a];
+      for (a = dim + 1; a < 3; a++) inner *= (size_t)L->nh[a];
+      buf = (char *)malloc(outer * h * inner * elem); /* sbuffer = slab.dup */
+      for (o = 0; o < outer; o++)
+        memcpy(buf + o * h * inner * elem, base + ((o * nh + src0) * inner) * elem,
+               h * inner * elem);
+      for (o = 0; o < outer; o++)
+        memcpy(base + ((o * nh + dst0) * inner) * elem, buf + o * h * inner * elem,
+               h * inner * elem);
       free(buf);
     }
   }
@@ -580,6 +570,7 @@ void or_advect_fields(or_lattice *L) {
     L->advection = L->fluids[f];
     L->fluids[f] = dst;
   }
+  L->density_fresh = 0; /* postAdvectionHooks: markDensitiesAsStale */
   L->timers[1] += now_s() - t0;
 }
 
@@ -587,6 +578,8 @@ void or_advect_fields(or_lattice *L) {
  * Every stored site, halo included. */
 void or_precalculate_densities(or_lattice *L) {
   int f;
+  if (L->density_fresh) return;
+  L->density_fresh = 1;
   for (f = 0; f < L->nc; f++) {
     const double *pop = L->fluids[f];
     double *rho = L->density[f];
@@ -883,6 +876,7 @@ void or_get_population(const or_lattice *L, int f, double *out) {
 }
 void or_set_population(or_lattice *L, int f, const double *in) {
   size_t o = 0;
+  L->density_fresh = 0;
   FOR_PHYS(L) {
     memcpy(L->fluids[f] + idx3(L, x, y, z) * L->q, in + o * L->q, sizeof(double) * L->q);
     o++;
@@ -1004,6 +998,7 @@ int or_init_fluid(or_lattice *L, int f, int kind, const double *dens, int ndens,
   double *fld = L->fluids[f];
   size_t s;
   int vq;
+  L->density_fresh = 0;
   if (reseed) { /* lb/init.d:113-121, shiftSeedByRank=false on one rank */
     or_mt_seed(&local, seed);
     g = &local;

@@ -1,0 +1,283 @@
+"""HIP path against the oracle, bit for bit (FP64), through the C ABI.
+
+Every test builds the same seeded input for the GPU (dlbc_amd) and for the
+oracle, advances both and compares populations with a bitwise equality: the
+kernels evaluate the reference's expressions in the reference's order
+without contraction, and exp is correctly rounded on both sides.
+"""
+import numpy as np
+import pytest
+
+from cases import assert_close, case, expand, golden, tolerance
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def D():
+    import dlbc_amd
+    import torch  # noqa: F401  (device visibility check)
+    if not torch.cuda.is_available():
+        pytest.fail("no GPU visible")
+    return dlbc_amd
+
+
+def _pair(D, oracle_mod, p, schedule=None):
+    g = D.Lattice.from_params(p, schedule=schedule)
+    o = oracle_mod.Oracle.from_params(p)
+    return g, o
+
+
+def _same_state(g, o, what):
+    for f in range(o.nc):
+        assert_close(g.population(f), o.population(f), 0.0, "%s population %d" % (what, f))
+
+
+def test_psi_exp_bitexact(D, oracle_mod):
+    """psi = 1 - exp(-rho) on the device equals the oracle's (both correctly
+    rounded) on 2^20 random densities plus edge values."""
+    L = D.Lattice(3, 19, 1, [4, 4, 4])
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.uniform(0.0, 3.0, 1 << 19), rng.uniform(-1.0, 40.0, 1 << 18),
+                        -rng.uniform(-745.0, 708.0, 1 << 18),
+                        np.array([0.0, -0.0, 1e-300, 5e-324, 708.39, 708.5, 745.1, 745.2,
+                                  746.0, -709.7, -710.0, np.inf, -np.inf])])
+    y = L.eval_psi(1, x)
+    ref = np.array([oracle_mod.psi_value(1, v) for v in x])
+    bad = np.nonzero(y.view(np.uint64) != ref.view(np.uint64))[0]
+    assert bad.size == 0, [(x[i], y[i], ref[i]) for i in bad[:5]]
+    assert np.array_equal(L.eval_psi(0, x[:100]), x[:100])
+
+
+@pytest.mark.parametrize("dq", [(3, 19), (2, 9), (1, 3)])
+def test_advect_bounce_back_bitexact(D, oracle_mod, dq):
+    """Streaming with bounce-back alone (advectFields) on random populations
+    and a random mask: identical to the oracle bit for bit, and equal to a
+    numpy restatement (roll, then bounce at solid upstream sites)."""
+    d, q = dq
+    n = {3: [9, 7, 12], 2: [17, 10], 1: [33]}[d]
+    rng = np.random.default_rng(d)
+    mask = (rng.random(n) < 0.2).astype(np.int32)
+    pops = [rng.uniform(0.0, 1.0, tuple(n) + (q,)) for _ in range(2)]
+    g = D.Lattice(d, q, 2, n)
+    g.set_params([1.0, 1.0])
+    g.upload_mask(mask)
+    for f in range(2):
+        g.upload_population(f, pops[f])
+    o = oracle_mod.Oracle(d, q, 2, n)
+    o.set_params([1.0, 1.0])
+    o.set_mask(mask)
+    for f in range(2):
+        o.set_population(f, pops[f])
+    o.exchange_halo()
+    o.advect()
+    g.advect()
+    _same_state(g, o, "advect")
+    vel, bounce, _, _ = oracle_mod.conn(d, q)
+    for f in range(2):
+        exp = np.empty_like(pops[f])
+        for i in range(q):
+            shifted = np.roll(pops[f][..., i], shift=tuple(vel[i]), axis=tuple(range(d)))
+            solid_up = np.roll(mask, shift=tuple(vel[i]), axis=tuple(range(d))) == 1
+            exp[..., i] = np.where(solid_up, pops[f][..., bounce[i]], shifted)
+        exp[mask == 1] = pops[f][mask == 1]
+        assert_close(g.population(f), exp, 0.0, "numpy streaming f%d" % f)
+
+
+STEP_CASES = [
+    ("bdist2_3d", {}, 10),
+    ("bdist2_2d", {}, 10),
+    ("bdist2_sc_3d", {"gn": [4, 4, 64]}, 100),
+    ("bdist2_sc_2d", {"gn": [64, 4]}, 100),
+    ("merging_droplets_2d", {}, 300),
+    ("poiseuille_2d", {}, 1000),
+    ("laplace_quasi_1d_d2q9", {}, 1000),
+    ("laplace_quasi_1d_d1q3", {}, 1000),
+    ("laplace_droplet_3d", {"gn": [24, 24, 24]}, 20),
+    ("checkpoint_3d", {}, 5),
+]
+
+
+@pytest.mark.parametrize("name,over,steps", STEP_CASES, ids=[c[0] for c in STEP_CASES])
+def test_reference_cases_bitexact(D, oracle_mod, name, over, steps):
+    """The reference's runnable-test configurations: GPU == oracle bitwise."""
+    p = case(name, **over)
+    g, o = _pair(D, oracle_mod, p)
+    g.step(steps)
+    o.run(steps)
+    _same_state(g, o, name)
+    for f in range(p["components"]):
+        assert_close(g.density(f), o.density(f), 0.0, "density")
+        assert_close(g.velocity(f), o.velocity(f), 0.0, "velocity")
+    if p["components"] == 2:
+        assert_close(g.colour(), o.colour(), 0.0, "colour")
+
+
+def _config1(n=64):
+    """D2Q9 single-component BGK, 64x64 periodic, tau 1, rho 1 +- 1e-3 seeded."""
+    return dict(d=2, q=9, gn=[n, n], components=1, tau=[1.0], fluidInit=["EqDistPerturb"],
+                fluidDensities=[[1.0]], fluidPerturb=[1e-3], randomReseed=[True],
+                randomSeed=[42])
+
+
+def _config2(n, form="SecondOrder"):
+    """D3Q19 two-component Shan-Chen spinodal decomposition, periodic,
+    tau 1, g 3.6, psi = 1 - exp(-rho), rho 0.7 +- 0.01 seeded per fluid."""
+    return dict(d=3, q=19, gn=[n, n, n], components=2, tau=[1.0, 1.0], eqDistForm=form,
+                gcc=[[0.0, 3.6], [3.6, 0.0]], enableShanChen=True, psiForm="Exponential",
+                fluidInit=["EqDistPerturb"] * 2, fluidDensities=[[0.7], [0.7]],
+                fluidPerturb=[0.01, 0.01], randomReseed=[True, True], randomSeed=[42, 43])
+
+
+def test_config1_d2q9_1000_steps_bitexact(D, oracle_mod):
+    p = _config1()
+    g, o = _pair(D, oracle_mod, p)
+    m0 = g.total_mass(0)
+    g.step(1000)
+    o.run(1000)
+    _same_state(g, o, "config1")
+    assert abs(g.total_mass(0) - m0) <= 1e-12 * m0
+
+
+@pytest.mark.parametrize("form", ["SecondOrder", "BDist2", "ThirdOrder"])
+def test_config2_spinodal_bitexact(D, oracle_mod, form):
+    """Spinodal decomposition amplifies any 1-ulp difference; 200 steps at
+    24^3 stay bit-identical."""
+    p = _config2(24, form)
+    g, o = _pair(D, oracle_mod, p)
+    g.step(200)
+    o.run(200)
+    _same_state(g, o, "config2 " + form)
+
+
+def test_walls_wetting_bodyforce_bitexact(D, oracle_mod):
+    """Walls with a wall-fluid coupling (gwc) and random solid sites, which
+    no reference fixture exercises: GPU == oracle.  (The oracle has no body
+    force; it is tested separately below.)"""
+    rng = np.random.default_rng(7)
+    n = [12, 10, 16]
+    mask = (rng.random(n) < 0.1).astype(np.int32)
+    mask[:, :, 0] = 1
+    mask[:, :, -1] = 1
+    p = dict(d=3, q=19, gn=n, components=2, tau=[0.8, 1.2], gcc=[[0.0, 3.6], [3.6, 0.0]],
+             gwc=[0.3, -0.2], enableShanChen=True, psiForm="Exponential",
+             globalAcc=[1e-5, 0.0, 0.0], fluidInit=["EqDistPerturb"] * 2,
+             fluidDensities=[[0.7], [0.3]], fluidPerturb=[0.05, 0.05],
+             randomReseed=[True, True], randomSeed=[5, 6], maskArray=mask)
+    for form in ("SecondOrder", "BDist2"):
+        p["eqDistForm"] = form
+        g, o = _pair(D, oracle_mod, p)
+        g.step(50)
+        o.run(50)
+        _same_state(g, o, "walls " + form)
+
+
+def test_body_force_equals_global_acceleration(D):
+    """On a uniform state a body force F_f = rho_f g and a global
+    acceleration g give the same velocity shift (up to the rounding of
+    (rho g)/rho), and a zero body force is bitwise a no-op."""
+    acc = [2e-5, 0.0, -1e-5]
+    p = dict(d=3, q=19, gn=[8, 8, 8], components=2, tau=[1.0, 0.8], eqDistForm="SecondOrder",
+             fluidInit=["EqDist"] * 2, fluidDensities=[[0.7], [0.3]])
+    a = D.Lattice.from_params(dict(p, globalAcc=acc))
+    b = D.Lattice.from_params(dict(p, bodyForce=[[0.7 * v for v in acc], [0.3 * v for v in acc]]))
+    c = D.Lattice.from_params(dict(p, bodyForce=[[0.0] * 3, [0.0] * 3]))
+    z = D.Lattice.from_params(p)
+    for L in (a, b, c, z):
+        L.step(1)
+    for f in range(2):
+        assert_close(b.population(f), a.population(f), 1e-17, "body force vs globalAcc")
+        assert not np.array_equal(a.population(f), z.population(f))
+        assert_close(c.population(f), z.population(f), 0.0, "zero body force")
+
+
+def test_mass_conservation_128(D):
+    """Config 2 at its full size, 1000 steps: total mass per component is
+    conserved to round-off (compensated device sum)."""
+    p = _config2(128)
+    g = D.Lattice.from_params(p)
+    m0 = [g.total_mass(f, parts=True) for f in range(2)]
+    g.step(1000)
+    for f in range(2):
+        m1 = g.total_mass(f, parts=True)
+        assert abs((m1[0] - m0[f][0]) + (m1[1] - m0[f][1])) <= 1e-11 * m0[f][0]
+    assert np.all(np.isfinite(g.density(0)))
+
+
+def test_total_mass_matches_exact_sum(D, oracle_mod):
+    """The device's double-double mass equals math.fsum of the per-site
+    densities (the exact sum) rounded to double."""
+    import math
+    p = _config2(20)
+    g, o = _pair(D, oracle_mod, p)
+    g.step(10)
+    o.run(10)
+    for f in range(2):
+        exact = math.fsum(o.density(f).ravel().tolist())
+        assert g.total_mass(f) == exact
+        mom = g.total_momentum(f)
+        assert np.allclose(mom, o.total_momentum(f), rtol=0, atol=1e-13)
+
+
+def test_laplace_droplet_3d_against_reference_t10000(D):
+    """lb/laplace-droplet-3d on the GPU for the full 10000 steps against the
+    reference's own t100 and t10000 data (central planes and exact totals),
+    tolerance 1e-15 as in its JSON."""
+    import math
+    gd, meta = golden("laplace_droplet_3d")
+    g = D.Lattice.from_params(case("laplace_droplet_3d"))
+    t = 0
+    for step in (100, 10000):
+        g.step(step - t)
+        t = step
+        for fname, arr in (("density_red", g.density(0)), ("density_blue", g.density(1)),
+                           ("colour_red_blue", g.colour())):
+            key = "%s_t%d" % (fname, step)
+            info = meta["fields"][key]
+            px, py, pz = info["planes"]
+            tol = tolerance(meta, step)
+            assert_close(arr[px], gd[key + "_x"], tol, key)
+            assert_close(arr[:, py], gd[key + "_y"], tol, key)
+            assert_close(arr[:, :, pz], gd[key + "_z"], tol, key)
+            assert abs(math.fsum(arr.ravel().tolist()) - info["fsum"]) <= tol * arr.size
+
+
+def test_merging_droplets_3d_against_reference_t1000(D):
+    import math
+    gd, meta = golden("merging_droplets_3d")
+    g = D.Lattice.from_params(case("merging_droplets_3d"))
+    g.step(1000)
+    for fname, arr in (("density_red", g.density(0)), ("density_blue", g.density(1)),
+                       ("colour_red_blue", g.colour())):
+        key = "%s_t1000" % fname
+        info = meta["fields"][key]
+        px, py, pz = info["planes"]
+        assert_close(arr[px], gd[key + "_x"], 1e-14, key)
+        assert_close(arr[:, py], gd[key + "_y"], 1e-14, key)
+        assert_close(arr[:, :, pz], gd[key + "_z"], 1e-14, key)
+        assert abs(math.fsum(arr.ravel().tolist()) - info["fsum"]) <= 1e-14 * arr.size
+
+
+def test_bdist2_sc_3d_full_size_against_reference(D):
+    """lb/bdist2-sc-3d at its real 64^3 size for 100 steps."""
+    gd, meta = golden("bdist2_sc_3d")
+    g = D.Lattice.from_params(case("bdist2_sc_3d"))
+    g.step(100)
+    for f, c in enumerate(("red", "blue")):
+        key = "population_%s_t100" % c
+        assert_close(g.population(f), expand(gd[key], meta["shapes"][key]), 1e-15, key)
+
+
+def test_errors_are_returned_not_fatal(D):
+    """Invalid calls return error codes with a message (the library never
+    exits): stepping before upload, a bad component, a bad mask value."""
+    L = D.Lattice(3, 19, 2, [8, 8, 8])
+    with pytest.raises(D.DlbcError, match="call out of order"):
+        L.step(1)
+    L.set_params([1.0, 1.0])
+    with pytest.raises(D.DlbcError):
+        L.upload_mask(np.full((8, 8, 8), 7, np.int32))
+    with pytest.raises(D.DlbcError):
+        L.population(5)
+    with pytest.raises(D.DlbcError):
+        D.Lattice(3, 19, 2, [1, 8, 8])
