@@ -30,6 +30,7 @@ struct dlbc_gpu_ctx {
   double *popA = nullptr, *popB = nullptr;  // FP64 storage
   double *S = nullptr;                      // psi (fluid) / rho (solid)
   uint8_t *mask = nullptr;
+  uint32_t *nbw = nullptr;  // solid-neighbour words (bit i: x + c_i solid)
   int has_solid = 0;
   bool params_set = false, pops_uploaded[kMaxC] = {false, false, false, false};
   bool solid_rho_dirty = true;
@@ -99,6 +100,24 @@ __global__ void k_soa_to_aos(const double *__restrict__ A, double *__restrict__ 
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= cnt) return;
   for (int i = 0; i < q; ++i) stg[(int64_t)t * q + i] = A[(int64_t)(f * q + i) * sp + s0 + t];
+}
+
+// Solid-neighbour word of every site: bit i = solid(x + c_i), bit 31 =
+// solid(x) (periodic neighbours).
+template <class L>
+__global__ void k_solid_words(const uint8_t *__restrict__ mask, uint32_t *__restrict__ nbw,
+                              Geo g) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= g.sites) return;
+  int x, y, z;
+  coords(s, g, x, y, z);
+  uint32_t w = (uint32_t)(mask[s] != 0) << 31;
+#pragma unroll
+  for (int i = 0; i < L::q; ++i)
+    w |= (uint32_t)(mask[site_of(wrap(x + L::c[i][0], g.nx), wrap(y + L::c[i][1], g.ny),
+                                 wrap(z + L::c[i][2], g.nz), g)] != 0)
+         << i;
+  nbw[s] = w;
 }
 
 template <class L>
@@ -294,7 +313,7 @@ struct FusedStep {
     (void)acc_ms;
     if constexpr (std::is_same<L, D3Q19>::value && C == 2) {
       tm->before();
-      const int rc = launch_fused_d3q19(c->popA, c->popB, c->mask, c->S, c->geo, c->p, c->stream);
+      const int rc = launch_fused_d3q19(c->popA, c->popB, c->nbw, c->S, c->geo, c->p, c->stream);
       tm->after();
       if (rc) return fail(c, DLBC_ERR_UNSUPPORTED, "fused kernel: lattice shape not supported");
       std::swap(c->popA, c->popB);
@@ -469,8 +488,11 @@ int dlbc_gpu_create(dlbc_gpu_ctx **out, int d, int q, int components, const int6
   if (hipMalloc(&c->S, sizeof(double) * components * c->geo.sp) != hipSuccess)
     return bail(DLBC_ERR_NOMEM);
   if (hipMalloc(&c->mask, c->geo.sp) != hipSuccess) return bail(DLBC_ERR_NOMEM);
-  c->dev_bytes = 2 * popb + sizeof(double) * components * c->geo.sp + c->geo.sp;
+  if (hipMalloc(&c->nbw, sizeof(uint32_t) * c->geo.sp) != hipSuccess) return bail(DLBC_ERR_NOMEM);
+  c->dev_bytes = 2 * popb + sizeof(double) * components * c->geo.sp + 5 * c->geo.sp;
   if (hipMemsetAsync(c->mask, 0, c->geo.sp, c->stream) != hipSuccess) return bail(DLBC_ERR_HIP);
+  if (hipMemsetAsync(c->nbw, 0, sizeof(uint32_t) * c->geo.sp, c->stream) != hipSuccess)
+    return bail(DLBC_ERR_HIP);
   if (hipMemsetAsync(c->S, 0, sizeof(double) * components * c->geo.sp, c->stream) != hipSuccess)
     return bail(DLBC_ERR_HIP);
   for (auto &e : c->ev)
@@ -526,6 +548,19 @@ int dlbc_gpu_upload_mask(dlbc_gpu_ctx *c, const int32_t *host, int halo) {
         solid |= v;
       }
   HIPCHK(c, hipMemcpyAsync(c->mask, m.data(), m.size(), hipMemcpyHostToDevice, c->stream));
+  {
+    const int nb = blocks_for(c->geo.sites, 256);
+    if (c->d == 3)
+      hipLaunchKernelGGL(k_solid_words<D3Q19>, dim3(nb), dim3(256), 0, c->stream, c->mask, c->nbw,
+                         c->geo);
+    else if (c->d == 2)
+      hipLaunchKernelGGL(k_solid_words<D2Q9>, dim3(nb), dim3(256), 0, c->stream, c->mask, c->nbw,
+                         c->geo);
+    else
+      hipLaunchKernelGGL(k_solid_words<D1Q3>, dim3(nb), dim3(256), 0, c->stream, c->mask, c->nbw,
+                         c->geo);
+    HIPCHK(c, hipGetLastError());
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->has_solid = solid;
   c->p.has_solid = solid;
@@ -731,6 +766,7 @@ void dlbc_gpu_destroy(dlbc_gpu_ctx *c) {
   if (c->popB) (void)hipFree(c->popB);
   if (c->S) (void)hipFree(c->S);
   if (c->mask) (void)hipFree(c->mask);
+  if (c->nbw) (void)hipFree(c->nbw);
   if (c->scratch) (void)hipFree(c->scratch);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/dlbc_hip.h"
+#include "lattice.hpp"
 
 namespace {
 
@@ -329,6 +330,42 @@ int dlbc_init_eq_dist_wall(double *host, const int32_t *mask, int d, int q, cons
       host[s * q] = 1.0;
     }
   return DLBC_OK;
+}
+
+int64_t dlbc_selftest_division(int64_t n, uint64_t seed) {
+  using dlbc::div_const;
+  const double css = dlbc::kCss, css2 = 2.0 * css, css2p2 = css2 * css;
+  const double css6p3 = 3.0 * css2p2 * css;
+  const double cs[4] = {css, css2, css2p2, css6p3};
+  uint64_t st = seed * 0x9E3779B97F4A7C15ull + 1;
+  auto next = [&]() {  // splitmix64
+    uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  };
+  int64_t bad = 0;
+  for (int64_t k = 0; k < n; ++k) {
+    const double c = cs[k & 3];
+    const uint64_t r = next();
+    double a;
+    uint64_t bits;
+    if ((k >> 2) & 1) {  // random mantissa, exponent in [-200, 200]
+      bits = (r & 0x000FFFFFFFFFFFFFull) | ((uint64_t)(1023 - 200 + (int)((r >> 52) % 401)) << 52);
+      std::memcpy(&a, &bits, 8);
+    } else {  // a few ulp from an exact multiple of the divisor
+      bits = (r & 0x000FFFFFFFFFFFFFull) | ((uint64_t)(1023 + (int)((r >> 52) % 64) - 32) << 52);
+      double m;
+      std::memcpy(&m, &bits, 8);
+      a = m * c;
+      std::memcpy(&bits, &a, 8);
+      bits += (int64_t)(next() % 9) - 4;
+      std::memcpy(&a, &bits, 8);
+    }
+    if (r >> 63) a = -a;
+    if (div_const(a, c, 1.0 / c) != a / c) ++bad;
+  }
+  return bad;
 }
 
 }  // extern "C"

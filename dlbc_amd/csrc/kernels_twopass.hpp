@@ -65,6 +65,107 @@ __device__ inline void pull_sources(int x, int y, int z, const Geo &g, const uin
   }
 }
 
+// ---- per-site physics, one component at a time ---------------------------
+// Both schedules call these, so they produce identical bits.
+
+// sum_i n_i c_i in i order, from 0 (momentum, lb/momentum.d:37-52; the
+// per-component sum of precalculateWeightedVelocity, lb/collision.d:271-276).
+template <class L>
+__device__ inline void first_moment(const double *n, double (&m)[L::d]) {
+#pragma unroll
+  for (int d = 0; d < L::d; ++d) m[d] = 0.0;
+#pragma unroll
+  for (int i = 0; i < L::q; ++i)
+#pragma unroll
+    for (int d = 0; d < L::d; ++d) m[d] += n[i] * static_cast<double>(L::c[i][d]);
+}
+
+// addShanChenForcePsi (src/dlbc/lb/force.d:238-309) for component f1 at one
+// collidable site, starting from F = 0 (resetForce).  psi(f, i): psi_f at the
+// neighbour x + c_i (i = -1: at the site); solid(i): x + c_i is solid;
+// rho_solid(f, i): rho_f at the solid neighbour x + c_i.
+template <class L, int C, class Psi, class Solid, class RhoSolid>
+__device__ inline void sc_force(int f1, double (&F)[L::d], const StepParams &p, Psi psi,
+                                Solid solid, RhoSolid rho_solid) {
+  constexpr int Q = L::q, D = L::d;
+#pragma unroll
+  for (int d = 0; d < D; ++d) F[d] = 0.0;
+  if (!p.sc) return;
+  const double psi1 = psi(f1, -1);
+#pragma unroll
+  for (int f2 = 0; f2 < C; ++f2) {
+    const double cc = p.gcc[f1][f2];
+    if (cc == 0.0) continue;
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {  // Iota!(1, q-1): i = 1 .. Q-1
+      const double psi2 = solid(i) ? psi(f2, -1) : psi(f2, i);
+      const double pf = L::w[i] * psi1 * psi2 * cc;
+#pragma unroll
+      for (int d = 0; d < D; ++d) F[d] -= pf * static_cast<double>(L::c[i][d]);
+    }
+  }
+  const double wc = p.gwc[f1];
+  if (wc != 0.0 && p.has_solid) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      if (solid(i)) {
+        const double pf = L::w[i] * psi1 * rho_solid(f1, i) * wc;
+#pragma unroll
+        for (int d = 0; d < D; ++d) F[d] -= pf * static_cast<double>(L::c[i][d]);
+      }
+    }
+  }
+}
+
+// totalDensity of distributeForce / precalculateWeightedVelocity.
+template <int C, class Rho>
+__device__ inline double total_density(Rho rho) {
+  double total = 0.0;
+#pragma unroll
+  for (int f = 0; f < C; ++f) total += rho(f);
+  return total;
+}
+
+// BDist2 weighted velocity (lb/collision.d:268-295) from the components'
+// first moments mom(f, d) and the total density.
+template <class L, int C, class Mom>
+__device__ inline void weighted_velocity(Mom mom, double total, const StepParams &p,
+                                         double (&wv)[L::d]) {
+#pragma unroll
+  for (int d = 0; d < L::d; ++d) wv[d] = 0.0;
+#pragma unroll
+  for (int f = 0; f < C; ++f)
+#pragma unroll
+    for (int d = 0; d < L::d; ++d) wv[d] += mom(f, d) / p.tau[f];
+#pragma unroll
+  for (int d = 0; d < L::d; ++d) wv[d] /= total;
+}
+
+// distributeForce (forceDistributed == 0, lb/force.d:179-197), the
+// per-component body force (a pre-collision hook) and the BGK collision
+// (lb/collision.d:90-119) of component f; out(i, value) stores the result.
+template <class L, class Out>
+__device__ inline void collide_component(int f, const double *n, double rho, double total,
+                                         double (&F)[L::d], const double *wv,
+                                         const StepParams &p, Out out) {
+  constexpr int Q = L::q, D = L::d;
+#pragma unroll
+  for (int d = 0; d < D; ++d) F[d] += 0.0 * (rho / total);
+  if (p.has_body) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) F[d] += p.body[f][d];
+  }
+  double dv[D], eq[Q];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    dv[d] = p.tau[f] * (p.gacc[d] + F[d] / rho);
+    if (p.eq_form == kBDist2) dv[d] += wv[d];
+  }
+  eq_dist<L>(p.eq_form, n, dv, rho, eq);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) out(i, n[i] * p.one_minus_omega[f] + p.omega[f] * eq[i]);
+}
+
 // advectFields alone (src/dlbc/lb/advection.d:35-89): A -> B for fluid sites;
 // solid sites hold the same populations in both buffers.
 template <class L, int C>
@@ -136,100 +237,35 @@ __global__ void __launch_bounds__(256) k_collide(const double *__restrict__ A,
   double rho[C];
 #pragma unroll
   for (int f = 0; f < C; ++f) rho[f] = density<L>(n[f]);
-
-  double F[C][D];
-#pragma unroll
-  for (int f = 0; f < C; ++f)
-#pragma unroll
-    for (int d = 0; d < D; ++d) F[f][d] = 0.0;  // resetForce
-
+  int nbr[Q];
+  bool solid_nb[Q];
   if (p.sc) {
-    int nbr[Q];
-    bool solid_nb[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       nbr[i] = site_of(wrap(x + L::c[i][0], g.nx), wrap(y + L::c[i][1], g.ny),
                        wrap(z + L::c[i][2], g.nz), g);
       solid_nb[i] = p.has_solid && mask[nbr[i]];
     }
-#pragma unroll
-    for (int f1 = 0; f1 < C; ++f1) {
-      const double psi1 = S[f1 * g.sp + s];
-#pragma unroll
-      for (int f2 = 0; f2 < C; ++f2) {
-        const double cc = p.gcc[f1][f2];
-        if (cc == 0.0) continue;
-#pragma unroll
-        for (int i = 1; i < Q; ++i) {  // Iota!(1, q-1): i = 1 .. Q-1
-          const double psi2 = solid_nb[i] ? S[f2 * g.sp + s] : S[f2 * g.sp + nbr[i]];
-          const double pf = L::w[i] * psi1 * psi2 * cc;
-#pragma unroll
-          for (int d = 0; d < D; ++d) F[f1][d] -= pf * static_cast<double>(L::c[i][d]);
-        }
-      }
-      const double wc = p.gwc[f1];
-      if (wc != 0.0 && p.has_solid) {
-#pragma unroll
-        for (int i = 0; i < Q; ++i) {
-          if (solid_nb[i]) {
-            const double pf = L::w[i] * psi1 * S[f1 * g.sp + nbr[i]] * wc;
-#pragma unroll
-            for (int d = 0; d < D; ++d) F[f1][d] -= pf * static_cast<double>(L::c[i][d]);
-          }
-        }
-      }
-    }
   }
-  {  // distributeForce with forceDistributed == 0
-    double total = 0.0;
+  double F[C][D];
 #pragma unroll
-    for (int f = 0; f < C; ++f) total += rho[f];
-#pragma unroll
-    for (int f = 0; f < C; ++f)
-#pragma unroll
-      for (int d = 0; d < D; ++d) F[f][d] += 0.0 * (rho[f] / total);
-  }
-  if (p.has_body) {  // per-component body force, applied as a pre-collision hook
-#pragma unroll
-    for (int f = 0; f < C; ++f)
-#pragma unroll
-      for (int d = 0; d < D; ++d) F[f][d] += p.body[f][d];
-  }
+  for (int f = 0; f < C; ++f)
+    sc_force<L, C>(
+        f, F[f], p, [&](int ff, int i) { return S[ff * g.sp + (i < 0 ? s : nbr[i])]; },
+        [&](int i) { return solid_nb[i]; }, [&](int ff, int i) { return S[ff * g.sp + nbr[i]]; });
+  const double total = total_density<C>([&](int f) { return rho[f]; });
   double wv[D];
   if (p.eq_form == kBDist2) {
+    double mom[C][D];
 #pragma unroll
-    for (int d = 0; d < D; ++d) wv[d] = 0.0;
-#pragma unroll
-    for (int f = 0; f < C; ++f) {
-      double per[D];
-#pragma unroll
-      for (int d = 0; d < D; ++d) per[d] = 0.0;
-#pragma unroll
-      for (int i = 0; i < Q; ++i)
-#pragma unroll
-        for (int d = 0; d < D; ++d) per[d] += n[f][i] * static_cast<double>(L::c[i][d]);
-#pragma unroll
-      for (int d = 0; d < D; ++d) wv[d] += per[d] / p.tau[f];
-    }
-    double total = 0.0;
-#pragma unroll
-    for (int f = 0; f < C; ++f) total += rho[f];
-#pragma unroll
-    for (int d = 0; d < D; ++d) wv[d] /= total;
+    for (int f = 0; f < C; ++f) first_moment<L>(n[f], mom[f]);
+    weighted_velocity<L, C>([&](int f, int d) { return mom[f][d]; }, total, p, wv);
   }
 #pragma unroll
-  for (int f = 0; f < C; ++f) {
-    double dv[D], eq[Q];
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      dv[d] = p.tau[f] * (p.gacc[d] + F[f][d] / rho[f]);
-      if (p.eq_form == kBDist2) dv[d] += wv[d];
-    }
-    eq_dist<L>(p.eq_form, n[f], dv, rho[f], eq);
-#pragma unroll
-    for (int i = 0; i < Q; ++i)
-      B[(int64_t)(f * Q + i) * g.sp + s] = n[f][i] * p.one_minus_omega[f] + p.omega[f] * eq[i];
-  }
+  for (int f = 0; f < C; ++f)
+    collide_component<L>(f, n[f], rho[f], total, F[f], wv, p, [&](int i, double v) {
+      B[(int64_t)(f * Q + i) * g.sp + s] = v;
+    });
 }
 
 }  // namespace dlbc

@@ -74,6 +74,21 @@ __host__ __device__ inline T density(const T *pop) {
   return T(0) + pairwise<0, L::q>(pop);
 }
 
+// RN(a / c) for a constant divisor c with rc = RN(1/c), by Markstein's
+// correction: q0 = RN(a rc) is within 1.5 ulp of a/c, one correction
+// q1 = RN(q0 + RN(a - q0 c) rc) makes it faithful, and for a faithful q1 the
+// remainder a - q1 c is exact and RN(q1 + (a - q1 c) rc) is the correctly
+// rounded quotient (Markstein's theorem).  Five fused operations instead of
+// the general IEEE division sequence.  Valid while a, a/c and the
+// intermediates are normal numbers; the equilibria below only divide
+// velocity moments (|a| << 2^1000) and a result below 2^-900 only ever enters
+// a sum with 1.0, where any value that small gives the same rounded sum.
+__host__ __device__ inline double div_const(double a, double c, double rc) {
+  const double q0 = a * rc;
+  const double q1 = fma(fma(-q0, c, a), rc, q0);
+  return fma(fma(-q1, c, a), rc, q1);
+}
+
 // eqDist (src/dlbc/lb/eqdist.d:61-278).  pop: the population, dv: the
 // velocity shift, rho0: its density.
 template <class L>
@@ -97,27 +112,35 @@ __device__ inline void eq_dist(int form, const double *pop, const double *dv, do
     double vdotv = 0.0;
 #pragma unroll
     for (int j = 0; j < D; ++j) vdotv += v[j] * v[j];
+    // c_s^2 products exactly as the reference forms them; every quotient
+    // by them is the correctly rounded one (div_const)
     constexpr double css = kCss;
-    const double css2 = 2.0 * css;
-    const double css2p2 = css2 * css;
+    constexpr double css2 = 2.0 * css;
+    constexpr double css2p2 = css2 * css;
+    constexpr double css6p3 = 3.0 * css2p2 * css;
+    constexpr double r_css = 1.0 / css, r_css2 = 1.0 / css2, r_css2p2 = 1.0 / css2p2,
+                     r_css6p3 = 1.0 / css6p3;
+    const double vv_css2 = div_const(vdotv, css2, r_css2);
     if (form == kSecondOrder) {  // eqdist.d:82-89
 #pragma unroll
       for (int i = 0; i < Q; ++i) {
         double vc = 0.0;
 #pragma unroll
         for (int j = 0; j < D; ++j) vc += v[j] * static_cast<double>(L::c[i][j]);
-        dist[i] = rho0 * L::w[i] * (1.0 + (vc / css) + (vc * vc / css2p2) - (vdotv / css2));
+        dist[i] = rho0 * L::w[i] *
+                  (1.0 + div_const(vc, css, r_css) + div_const(vc * vc, css2p2, r_css2p2) -
+                   vv_css2);
       }
     } else {  // ThirdOrder, eqdist.d:90-100
-      const double css6p3 = 3.0 * css2p2 * css;
 #pragma unroll
       for (int i = 0; i < Q; ++i) {
         double vc = 0.0;
 #pragma unroll
         for (int j = 0; j < D; ++j) vc += v[j] * static_cast<double>(L::c[i][j]);
         dist[i] = rho0 * L::w[i] *
-                  (1.0 + (vc / css) + (vc * vc / css2p2) - (vdotv / css2) +
-                   (vc * vc * vc / css6p3) - (vdotv * vc / css2p2));
+                  (1.0 + div_const(vc, css, r_css) + div_const(vc * vc, css2p2, r_css2p2) -
+                   vv_css2 + div_const(vc * vc * vc, css6p3, r_css6p3) -
+                   div_const(vdotv * vc, css2p2, r_css2p2));
       }
     }
     return;

@@ -150,6 +150,13 @@ int dlbc_gpu_total_momentum(dlbc_gpu_ctx *ctx, int f, double *out /*2d*/);
  * correctly rounded exp).  x and y are host arrays. */
 int dlbc_gpu_eval_psi(dlbc_gpu_ctx *ctx, int psi_form, const double *x, double *y, int64_t n);
 
+/* Self-check of the exact quotient by the constant c_s^2 products that the
+ * equilibria use in place of IEEE division (same code on host and device):
+ * compares it with IEEE division on n pseudo-random operands, half of them
+ * next to exact multiples of a divisor, and returns the number of mismatches
+ * (0 expected).  Host only, no GPU needed. */
+int64_t dlbc_selftest_division(int64_t n, uint64_t seed);
+
 const char *dlbc_gpu_last_error(const dlbc_gpu_ctx *ctx);
 void dlbc_gpu_destroy(dlbc_gpu_ctx *ctx);
 

@@ -108,3 +108,12 @@ def test_no_cpu_fallback(monkeypatch):
     monkeypatch.setattr(lat, "_LIB", "/nonexistent/libdlbc_hip.so")
     with pytest.raises(lat.DlbcError, match="not built"):
         lat.lib()
+
+
+def test_exact_constant_division_selftest():
+    """The equilibria divide by the c_s^2 products with an FMA-based exact
+    quotient (Markstein's correction) instead of IEEE division; on 4 million
+    operands, half of them within 4 ulp of an exact multiple of a divisor,
+    it equals IEEE division bit for bit."""
+    import dlbc_amd as D
+    assert D.lib().dlbc_selftest_division(4_000_000, 2026) == 0
