@@ -5,7 +5,11 @@ Workload (default): D3Q19 two-component Shan-Chen channel, 256^3, solid walls
 at z = 0 and 255 plus seeded random solid spheres (radius 4..12, solid
 fraction 0.2) with bounce-back, body force 1e-6 along x on component 1,
 red/blue halves (EqDistLamellae along x: red 1.0/0.05, blue 0.05/1.0),
-tau 1, g = 3.6, psi = 1 - exp(-rho), SecondOrder equilibrium, FP64.
+tau 1, g = 3.6, psi = 1 - exp(-rho), BDist2 equilibrium, FP64.  (With the
+SecondOrder or ThirdOrder equilibrium this workload diverges within 100
+steps in the reference's own scheme; BDist2 is the form of all the
+reference's Shan-Chen tests.)  The same workload is also timed with the
+FP32- and FP16-shifted population storages.
 
 Prints one JSON line: MLUPS (sites counted once, as the reference's LUPS),
 the roofline of the dominant kernel (algorithmic bytes per launch over its
@@ -61,7 +65,7 @@ def channel_mask(n, fraction=0.2, seed=20261202):
 
 def channel_params(n):
     return dict(d=3, q=19, gn=[n, n, n], components=2, tau=[1.0, 1.0],
-                eqDistForm="SecondOrder", gcc=[[0.0, 3.6], [3.6, 0.0]], enableShanChen=True,
+                eqDistForm="BDist2", gcc=[[0.0, 3.6], [3.6, 0.0]], enableShanChen=True,
                 psiForm="Exponential", bodyForce=[[0.0, 0.0, 0.0], [1e-6, 0.0, 0.0]],
                 fluidInit=["EqDistLamellae"] * 2, fluidDensities=[[1.0, 0.05], [0.05, 1.0]],
                 lamellaeWidths=[n // 2, n // 2], initAxis="X")
@@ -79,10 +83,10 @@ def build_inputs(D, p, mask):
     return pops
 
 
-def make_lattice(D, p, mask, pops, schedule):
-    L = D.Lattice(3, Q, 2, p["gn"])
+def make_lattice(D, p, mask, pops, schedule, storage="f64", rho0=None):
+    L = D.Lattice(3, Q, 2, p["gn"], storage=storage)
     L.set_params(p["tau"], p["gcc"], None, None, p["bodyForce"], p["eqDistForm"], p["psiForm"],
-                 True)
+                 True, rho0)
     if schedule != "auto":
         L.set_schedule(schedule)
     L.upload_mask(mask)
@@ -91,13 +95,52 @@ def make_lattice(D, p, mask, pops, schedule):
     return L
 
 
+def alg_bytes(fluid, sites, elem):
+    """Populations read and written once per fluid site, one mask byte per site."""
+    return fluid * 2 * C * Q * elem + sites
+
+
+def time_steps(L, steps, warmup):
+    """Wall time of `steps` steps bracketed by device synchronisations."""
+    import torch
+    L.step(warmup)
+    L.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    L.step(steps)
+    L.synchronize()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def storage_leg(D, p, mask, pops, storage, steps, warmup, fluid):
+    """MLUPS and roofline of the same workload with shifted storage."""
+    rho0 = [0.525, 0.525]  # midpoint of the two bulk densities
+    L = make_lattice(D, p, mask, pops, "auto", storage, rho0)
+    m0 = [L.total_mass(f) for f in range(2)]
+    dt = time_steps(L, steps, warmup)
+    sites = int(np.prod(p["gn"]))
+    el, kern = L.step_timed(max(5, steps // 2))
+    dom = max(kern, key=kern.get)
+    b = alg_bytes(fluid, sites, {"f32_shift": 4, "f16_shift": 2}[storage])
+    drift = max(abs(L.total_mass(f) - m0[f]) / m0[f] for f in range(2))
+    out = {"value": round(sites * steps / dt / 1e6, 2), "unit": "MLUPS",
+           "ms_per_step": round(1e3 * dt / steps, 4), "kernel": dom,
+           "launch_ms": round(kern[dom], 4), "bytes_per_launch": b,
+           "achieved": round(b / (kern[dom] * 1e-3) / 1e9, 1),
+           "frac": round(b / (kern[dom] * 1e-3) / HBM_PEAK, 4), "rho0": rho0,
+           "mass_rel_drift": float("%.3g" % drift)}
+    del L
+    return out
+
+
 def cpu_baseline(p, mask, pops, steps=2):
     """The oracle (C restatement of the reference's passes) on the host cores."""
     from oracle import oracle as OR
     OR.build()
     n = p["gn"]
     o = OR.Oracle(3, Q, 2, n)
-    o.set_params(p["tau"], p["gcc"], None, None, 0, 1, True)
+    o.set_params(p["tau"], p["gcc"], None, None, 2, 1, True)
     o.set_mask(mask)
     for f in range(2):
         o.set_population(f, pops[f])
@@ -114,14 +157,37 @@ def cpu_baseline(p, mask, pops, steps=2):
                       "plugin hook, is left out of this leg" % (steps, "x".join(map(str, n)))}
 
 
+def copy_calibration(gib=2.0, reps=5):
+    """A plain streaming kernel of known traffic (torch.neg over FP64 of
+    `gib` GiB: every byte read once and written once), for scaling the
+    FETCH_SIZE / WRITE_SIZE counters and as the measured copy bandwidth."""
+    import torch
+    n = int(gib * (1 << 30)) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    print(json.dumps({"copy_kernel": "torch.neg fp64", "bytes_read": 8 * n, "bytes_written": 8 * n,
+                      "ms": round(dt * 1e3, 4), "GBps_read_plus_write": round(16 * n / dt / 1e9, 1)}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--schedule", default="auto", choices=["auto", "two_pass", "fused"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-storage-legs", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="time a plain FP64 streaming kernel of known size and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
@@ -131,6 +197,9 @@ def main():
     import dlbc_amd as D
     if not torch.cuda.is_available():
         sys.exit("bench.py: no GPU visible")
+    if args.copy_calibration:
+        copy_calibration()
+        return
     n = args.size
     p = channel_params(n)
     t0 = time.perf_counter()
@@ -139,14 +208,8 @@ def main():
     log("inputs %.1f s, solid fraction %.4f" % (time.perf_counter() - t0, mask.mean()))
     L = make_lattice(D, p, mask, pops, args.schedule)
     log("uploaded; schedule %s; device bytes %.2f GB" % (L.schedule, L.device_bytes() / 1e9))
-    L.step(args.warmup)
-    L.synchronize()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    L.step(args.steps)
-    L.synchronize()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    m0 = [L.total_mass(f) for f in range(2)]
+    dt = time_steps(L, args.steps, args.warmup)
     sites = n ** 3
     mlups = sites * args.steps / dt / 1e6
     log("timed %d steps: %.3f ms/step, %.1f MLUPS" % (args.steps, 1e3 * dt / args.steps, mlups))
@@ -154,29 +217,41 @@ def main():
     el, kern = L.step_timed(max(5, args.steps // 2))
     dom = max(kern, key=kern.get)
     fluid = int(sites - mask.sum())
-    b_alg = fluid * 2 * C * Q * 8 + sites * 1  # populations read+written once, mask byte
+    b_alg = alg_bytes(fluid, sites, 8)
     achieved = b_alg / (kern[dom] * 1e-3)
     log("kernels (ms/launch): %s" % kern)
+    m1 = [L.total_mass(f) for f in range(2)]
+    drift = max(abs(m1[f] - m0[f]) / m0[f] for f in range(2))
+    if not all(np.isfinite(m1)):
+        sys.exit("bench.py: the state is not finite after the timed steps")
     result = {
-        "metric": "MLUPS, D3Q19 2-fluid SC 256^3 (FP64 plain storage)",
+        "metric": "MLUPS + % HBM peak, D3Q19 2-fluid SC 256\u00b3: FP64 vs FP32/FP16-shifted "
+                  "storage",
         "value": round(mlups, 2), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(1e3 * dt / args.steps, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "D3Q19 two-component Shan-Chen channel %d^3, walls z=0,%d + "
-                               "random spheres r 4-12 (solid fraction %.3f), body force 1e-6 x "
-                               "on component 1, SecondOrder, tau 1, g 3.6, FP64"
-                               % (n, n - 1, float(mask.mean())),
-                   "schedule": L.schedule, "sites": sites, "fluid_sites": fluid},
+                               "random spheres r 4-12 (solid fraction %.3f), red/blue halves "
+                               "1.0/0.05, body force 1e-6 x on component 1, BDist2, tau 1, "
+                               "g 3.6, FP64" % (n, n - 1, float(mask.mean())),
+                   "schedule": L.schedule, "sites": sites, "fluid_sites": fluid,
+                   "mass_rel_drift": float("%.3g" % drift)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 1),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
                      "bytes_per_launch": b_alg, "launch_ms": round(kern[dom], 4),
                      "step_frac": round(mlups * 1e6 * (b_alg / sites) / HBM_PEAK, 4)},
     }
+    del L
+    torch.cuda.empty_cache()
+    if not args.no_storage_legs:
+        result["storage_legs"] = {}
+        for st in ("f32_shift", "f16_shift"):
+            leg = storage_leg(D, p, mask, pops, st, args.steps, args.warmup, fluid)
+            log("%s: %s" % (st, leg))
+            result["storage_legs"][st] = leg
     if not args.no_cpu_baseline:
-        del L
-        torch.cuda.empty_cache()
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(p, mask, pops)
     print(json.dumps(result), flush=True)

@@ -11,6 +11,7 @@
 
 #include "../../include/dlbc_hip.h"
 #include "kernels_fused.hpp"
+#include "kernels_shifted.hpp"
 #include "kernels_twopass.hpp"
 
 using namespace dlbc;
@@ -27,7 +28,10 @@ struct dlbc_gpu_ctx {
   int n[3] = {1, 1, 1};
   Geo geo{};
   hipStream_t stream = nullptr;
-  double *popA = nullptr, *popB = nullptr;  // FP64 storage
+  void *popA = nullptr, *popB = nullptr;  // populations (double / float / __half)
+  int elem = 8;                            // bytes per stored population
+  double rho0[kMaxC] = {1.0, 1.0, 1.0, 1.0};
+  ShiftParams sparams{};
   double *S = nullptr;                      // psi (fluid) / rho (solid)
   uint8_t *mask = nullptr;
   uint32_t *nbw = nullptr;  // solid-neighbour words (bit i: x + c_i solid)
@@ -36,6 +40,7 @@ struct dlbc_gpu_ctx {
   bool solid_rho_dirty = true;
   StepParams p{};
   int schedule = DLBC_SCHED_AUTO;
+  int target_blocks = 0;  // fused kernels: minimum workgroups per launch (0: default)
   std::string err;
   int64_t dev_bytes = 0;
   hipEvent_t ev[2 * kMaxKernels + 2] = {};
@@ -82,24 +87,54 @@ int ensure_scratch(dlbc_gpu_ctx *c, size_t bytes) {
 
 inline int blocks_for(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
 
+// ---- population loaders -----------------------------------------------------
+// A loader returns population i of component f at site s in FP64: the stored
+// value (FP64 storage) or w_i rho0_f + the stored deviation (shifted storage).
+struct LoadF64 {
+  const double *A;
+  int64_t sp;
+  int q;
+  __device__ double operator()(int f, int i, int s) const {
+    return A[(int64_t)(f * q + i) * sp + s];
+  }
+};
+template <class St>
+struct LoadShift {
+  const St *A;
+  int64_t sp;
+  double rho0[4];
+  __device__ double operator()(int f, int i, int s) const {
+    return D3Q19::w[i] * rho0[f] + (double)to_f(A[(int64_t)(f * 19 + i) * sp + s]);
+  }
+};
+
 // ---- layout conversion kernels ---------------------------------------------
 // AoS staging [s][q] (physical sites s0.. s0+cnt) <-> SoA component f.
-__global__ void k_aos_to_soa(const double *__restrict__ stg, double *__restrict__ A,
-                             double *__restrict__ B, int q, int f, int64_t sp, int s0, int cnt) {
+// Shifted storage keeps d_i = f_i - w_i rho0 rounded to the storage type.
+template <class St>
+__global__ void k_aos_to_soa(const double *__restrict__ stg, St *__restrict__ A,
+                             St *__restrict__ B, int q, int f, int64_t sp, int s0, int cnt,
+                             double rho0) {
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= cnt) return;
   for (int i = 0; i < q; ++i) {
-    const double v = stg[(int64_t)t * q + i];
-    A[(int64_t)(f * q + i) * sp + s0 + t] = v;
-    B[(int64_t)(f * q + i) * sp + s0 + t] = v;
+    double v = stg[(int64_t)t * q + i];
+    St sv;
+    if constexpr (std::is_same<St, double>::value) {
+      sv = v;
+    } else {
+      sv = from_f<St>((float)(v - D3Q19::w[i] * rho0));
+    }
+    A[(int64_t)(f * q + i) * sp + s0 + t] = sv;
+    B[(int64_t)(f * q + i) * sp + s0 + t] = sv;
   }
 }
 
-__global__ void k_soa_to_aos(const double *__restrict__ A, double *__restrict__ stg, int q, int f,
-                             int64_t sp, int s0, int cnt) {
+template <class Ld>
+__global__ void k_soa_to_aos(Ld ld, double *__restrict__ stg, int q, int f, int s0, int cnt) {
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= cnt) return;
-  for (int i = 0; i < q; ++i) stg[(int64_t)t * q + i] = A[(int64_t)(f * q + i) * sp + s0 + t];
+  for (int i = 0; i < q; ++i) stg[(int64_t)t * q + i] = ld(f, i, s0 + t);
 }
 
 // Solid-neighbour word of every site: bit i = solid(x + c_i), bit 31 =
@@ -120,15 +155,15 @@ __global__ void k_solid_words(const uint8_t *__restrict__ mask, uint32_t *__rest
   nbw[s] = w;
 }
 
-template <class L>
-__global__ void k_solid_rho(const double *__restrict__ A, const uint8_t *__restrict__ mask,
-                            double *__restrict__ S, int C, Geo g) {
+template <class L, class Ld>
+__global__ void k_solid_rho(Ld ld, const uint8_t *__restrict__ mask, double *__restrict__ S,
+                            int C, Geo g) {
   const int s = blockIdx.x * 256 + threadIdx.x;
   if (s >= g.sites || !mask[s]) return;
   for (int f = 0; f < C; ++f) {
     double n[L::q];
 #pragma unroll
-    for (int i = 0; i < L::q; ++i) n[i] = A[(int64_t)(f * L::q + i) * g.sp + s];
+    for (int i = 0; i < L::q; ++i) n[i] = ld(f, i, s);
     S[f * g.sp + s] = density<L>(n);
   }
 }
@@ -136,14 +171,14 @@ __global__ void k_solid_rho(const double *__restrict__ A, const uint8_t *__restr
 // Output fields (src/dlbc/lb/io.d:56-101).  kind 0: density of f1 (all sites);
 // 1: raw velocity of f1 (lb/velocity.d:37-48, 106-117; 0 on solid);
 // 2: colour rho_f1 - rho_f2 (lb/density.d:284-295; 0 on solid).
-template <class L>
-__global__ void k_field(const double *__restrict__ A, const uint8_t *__restrict__ mask,
-                        double *__restrict__ out, Geo g, int kind, int f1, int f2, int has_solid) {
+template <class L, class Ld>
+__global__ void k_field(Ld ld, const uint8_t *__restrict__ mask, double *__restrict__ out, Geo g,
+                        int kind, int f1, int f2, int has_solid) {
   const int s = blockIdx.x * 256 + threadIdx.x;
   if (s >= g.sites) return;
   double n[L::q];
 #pragma unroll
-  for (int i = 0; i < L::q; ++i) n[i] = A[(int64_t)(f1 * L::q + i) * g.sp + s];
+  for (int i = 0; i < L::q; ++i) n[i] = ld(f1, i, s);
   const bool solid = has_solid && mask[s];
   if (kind == 0) {
     out[s] = density<L>(n);
@@ -168,7 +203,7 @@ __global__ void k_field(const double *__restrict__ A, const uint8_t *__restrict_
     } else {
       double m[L::q];
 #pragma unroll
-      for (int i = 0; i < L::q; ++i) m[i] = A[(int64_t)(f2 * L::q + i) * g.sp + s];
+      for (int i = 0; i < L::q; ++i) m[i] = ld(f2, i, s);
       out[s] = density<L>(n) - density<L>(m);
     }
   }
@@ -179,9 +214,8 @@ __global__ void k_field(const double *__restrict__ A, const uint8_t *__restrict_
 // one block sums the block partials the same way.  what: 0 mass, 1..3 momentum.
 __device__ inline DD dd_plus(DD a, DD b) { return dd_add(a, b); }
 
-template <class L>
-__global__ void __launch_bounds__(256) k_reduce1(const double *__restrict__ A,
-                                                 const uint8_t *__restrict__ mask,
+template <class L, class Ld>
+__global__ void __launch_bounds__(256) k_reduce1(Ld ld, const uint8_t *__restrict__ mask,
                                                  double *__restrict__ part, Geo g, int f,
                                                  int what, int chunk, int has_solid) {
   __shared__ double sh[256], sl[256];
@@ -193,7 +227,7 @@ __global__ void __launch_bounds__(256) k_reduce1(const double *__restrict__ A,
     if (has_solid && mask[s]) continue;
     double n[L::q];
 #pragma unroll
-    for (int i = 0; i < L::q; ++i) n[i] = A[(int64_t)(f * L::q + i) * g.sp + s];
+    for (int i = 0; i < L::q; ++i) n[i] = ld(f, i, s);
     double v;
     if (what == 0) {
       v = density<L>(n);
@@ -291,15 +325,17 @@ template <class L, int C>
 struct TwoPassStep {
   static int run(dlbc_gpu_ctx *c, Timer *tm, double *acc_ms) {
     const int nb = blocks_for(c->geo.sites, 256);
+    const double *A = static_cast<const double *>(c->popA);
+    double *B = static_cast<double *>(c->popB);
     if (c->p.sc) {
       tm->before();
-      hipLaunchKernelGGL((k_psi<L, C>), dim3(nb), dim3(256), 0, c->stream, c->popA, c->mask,
-                         c->S, c->geo, c->p.psi_form, c->has_solid);
+      hipLaunchKernelGGL((k_psi<L, C>), dim3(nb), dim3(256), 0, c->stream, A, c->mask, c->S,
+                         c->geo, c->p.psi_form, c->has_solid);
       tm->after();
     }
     tm->before();
-    hipLaunchKernelGGL((k_collide<L, C>), dim3(nb), dim3(256), 0, c->stream, c->popA, c->popB,
-                       c->mask, c->S, c->geo, c->p);
+    hipLaunchKernelGGL((k_collide<L, C>), dim3(nb), dim3(256), 0, c->stream, A, B, c->mask, c->S,
+                       c->geo, c->p);
     tm->after();
     (void)acc_ms;
     std::swap(c->popA, c->popB);
@@ -313,7 +349,17 @@ struct FusedStep {
     (void)acc_ms;
     if constexpr (std::is_same<L, D3Q19>::value && C == 2) {
       tm->before();
-      const int rc = launch_fused_d3q19(c->popA, c->popB, c->nbw, c->S, c->geo, c->p, c->stream);
+      int rc;
+      if (c->storage == DLBC_STORAGE_F64)
+        rc = launch_fused_d3q19(static_cast<const double *>(c->popA),
+                                static_cast<double *>(c->popB), c->nbw, c->S, c->geo, c->p,
+                                c->target_blocks, c->stream);
+      else if (c->storage == DLBC_STORAGE_F32_SHIFT)
+        rc = launch_shifted_d3q19<float>(c->popA, c->popB, c->nbw, c->S, c->geo, c->sparams,
+                                         c->target_blocks, c->stream);
+      else
+        rc = launch_shifted_d3q19<__half>(c->popA, c->popB, c->nbw, c->S, c->geo, c->sparams,
+                                          c->target_blocks, c->stream);
       tm->after();
       if (rc) return fail(c, DLBC_ERR_UNSUPPORTED, "fused kernel: lattice shape not supported");
       std::swap(c->popA, c->popB);
@@ -327,14 +373,37 @@ struct FusedStep {
 template <class L, int C>
 struct AdvectOnly {
   static int run(dlbc_gpu_ctx *c) {
+    if (c->storage != DLBC_STORAGE_F64)
+      return fail(c, DLBC_ERR_UNSUPPORTED, "dlbc_gpu_advect needs FP64 storage");
     hipLaunchKernelGGL((k_advect<L, C>), dim3(blocks_for(c->geo.sites, 256)), dim3(256), 0,
-                       c->stream, c->popA, c->popB, c->mask, c->geo, c->has_solid);
+                       c->stream, static_cast<const double *>(c->popA),
+                       static_cast<double *>(c->popB), c->mask, c->geo, c->has_solid);
     std::swap(c->popA, c->popB);
     return DLBC_OK;
   }
 };
 
+// Call fn(lattice, loader) with the loader of the context's storage.
+template <class Fn>
+int with_loader(dlbc_gpu_ctx *c, Fn fn) {
+  if (c->storage == DLBC_STORAGE_F64) {
+    const LoadF64 ld{static_cast<const double *>(c->popA), c->geo.sp, c->q};
+    if (c->d == 3) return fn(D3Q19{}, ld);
+    if (c->d == 2) return fn(D2Q9{}, ld);
+    return fn(D1Q3{}, ld);
+  }
+  if (c->storage == DLBC_STORAGE_F32_SHIFT) {
+    const LoadShift<float> ld{static_cast<const float *>(c->popA), c->geo.sp,
+                              {c->rho0[0], c->rho0[1], c->rho0[2], c->rho0[3]}};
+    return fn(D3Q19{}, ld);
+  }
+  const LoadShift<__half> ld{static_cast<const __half *>(c->popA), c->geo.sp,
+                             {c->rho0[0], c->rho0[1], c->rho0[2], c->rho0[3]}};
+  return fn(D3Q19{}, ld);
+}
+
 int effective_schedule(const dlbc_gpu_ctx *c) {
+  if (c->storage != DLBC_STORAGE_F64) return DLBC_SCHED_FUSED;
   if (c->schedule != DLBC_SCHED_AUTO) return c->schedule;
   if (c->d == 3 && c->C == 2 && fused_d3q19_supported(c->geo)) return DLBC_SCHED_FUSED;
   return DLBC_SCHED_TWO_PASS;
@@ -347,15 +416,11 @@ int prepare(dlbc_gpu_ctx *c) {
   if (c->solid_rho_dirty) {
     if (c->has_solid) {
       const int nb = blocks_for(c->geo.sites, 256);
-      if (c->d == 3)
-        hipLaunchKernelGGL(k_solid_rho<D3Q19>, dim3(nb), dim3(256), 0, c->stream, c->popA,
-                           c->mask, c->S, c->C, c->geo);
-      else if (c->d == 2)
-        hipLaunchKernelGGL(k_solid_rho<D2Q9>, dim3(nb), dim3(256), 0, c->stream, c->popA,
-                           c->mask, c->S, c->C, c->geo);
-      else
-        hipLaunchKernelGGL(k_solid_rho<D1Q3>, dim3(nb), dim3(256), 0, c->stream, c->popA,
-                           c->mask, c->S, c->C, c->geo);
+      with_loader(c, [&](auto lat, auto ld) {
+        hipLaunchKernelGGL((k_solid_rho<decltype(lat), decltype(ld)>), dim3(nb), dim3(256), 0,
+                           c->stream, ld, c->mask, c->S, c->C, c->geo);
+        return 0;
+      });
       HIPCHK(c, hipGetLastError());
     }
     c->solid_rho_dirty = false;
@@ -407,15 +472,11 @@ int download_sitefield(dlbc_gpu_ctx *c, const double *dev, int k, double *host, 
 
 int field_launch(dlbc_gpu_ctx *c, double *out, int kind, int f1, int f2) {
   const int nb = blocks_for(c->geo.sites, 256);
-  if (c->d == 3)
-    hipLaunchKernelGGL(k_field<D3Q19>, dim3(nb), dim3(256), 0, c->stream, c->popA, c->mask, out,
-                       c->geo, kind, f1, f2, c->has_solid);
-  else if (c->d == 2)
-    hipLaunchKernelGGL(k_field<D2Q9>, dim3(nb), dim3(256), 0, c->stream, c->popA, c->mask, out,
-                       c->geo, kind, f1, f2, c->has_solid);
-  else
-    hipLaunchKernelGGL(k_field<D1Q3>, dim3(nb), dim3(256), 0, c->stream, c->popA, c->mask, out,
-                       c->geo, kind, f1, f2, c->has_solid);
+  with_loader(c, [&](auto lat, auto ld) {
+    hipLaunchKernelGGL((k_field<decltype(lat), decltype(ld)>), dim3(nb), dim3(256), 0, c->stream,
+                       ld, c->mask, out, c->geo, kind, f1, f2, c->has_solid);
+    return 0;
+  });
   HIPCHK(c, hipGetLastError());
   return DLBC_OK;
 }
@@ -426,15 +487,11 @@ int reduce_launch(dlbc_gpu_ctx *c, int f, int what, double *out2) {
   int rc = ensure_scratch(c, sizeof(double) * (2 * (size_t)nparts + 2));
   if (rc) return rc;
   double *part = c->scratch, *res = c->scratch + 2 * nparts;
-  if (c->d == 3)
-    hipLaunchKernelGGL(k_reduce1<D3Q19>, dim3(nparts), dim3(256), 0, c->stream, c->popA, c->mask,
-                       part, c->geo, f, what, chunk, c->has_solid);
-  else if (c->d == 2)
-    hipLaunchKernelGGL(k_reduce1<D2Q9>, dim3(nparts), dim3(256), 0, c->stream, c->popA, c->mask,
-                       part, c->geo, f, what, chunk, c->has_solid);
-  else
-    hipLaunchKernelGGL(k_reduce1<D1Q3>, dim3(nparts), dim3(256), 0, c->stream, c->popA, c->mask,
-                       part, c->geo, f, what, chunk, c->has_solid);
+  with_loader(c, [&](auto lat, auto ld) {
+    hipLaunchKernelGGL((k_reduce1<decltype(lat), decltype(ld)>), dim3(nparts), dim3(256), 0,
+                       c->stream, ld, c->mask, part, c->geo, f, what, chunk, c->has_solid);
+    return 0;
+  });
   hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(256), 0, c->stream, part, nparts, res);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(out2, res, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -456,7 +513,9 @@ int dlbc_gpu_create(dlbc_gpu_ctx **out, int d, int q, int components, const int6
   if (!((d == 3 && q == 19) || (d == 2 && q == 9) || (d == 1 && q == 3)))
     return DLBC_ERR_UNSUPPORTED;
   if (components < 1 || components > kMaxC) return DLBC_ERR_UNSUPPORTED;
-  if (storage != DLBC_STORAGE_F64) return DLBC_ERR_UNSUPPORTED;
+  if (storage < DLBC_STORAGE_F64 || storage > DLBC_STORAGE_F16_SHIFT) return DLBC_ERR_ARG;
+  if (storage != DLBC_STORAGE_F64 && !(d == 3 && q == 19 && components == 2))
+    return DLBC_ERR_UNSUPPORTED;  // shifted storage: the D3Q19 two-component kernel only
   int64_t sites = 1;
   for (int i = 0; i < d; ++i) {
     if (n[i] < 2 || n[i] > (1 << 20)) return DLBC_ERR_ARG;
@@ -479,10 +538,14 @@ int dlbc_gpu_create(dlbc_gpu_ctx **out, int d, int q, int components, const int6
     dlbc_gpu_destroy(c);
     return code;
   };
+  // shifted storage runs on the fused D3Q19 kernel only: n_y % 8 == 0, n_z % 32 == 0
+  if (storage != DLBC_STORAGE_F64 && !fused_d3q19_supported(c->geo))
+    return bail(DLBC_ERR_UNSUPPORTED);
+  c->elem = storage == DLBC_STORAGE_F64 ? 8 : (storage == DLBC_STORAGE_F32_SHIFT ? 4 : 2);
   if (hipSetDevice(device) != hipSuccess) return bail(DLBC_ERR_HIP);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(DLBC_ERR_HIP);
-  const size_t popb = sizeof(double) * (size_t)components * q * c->geo.sp;
+  const size_t popb = (size_t)c->elem * components * q * c->geo.sp;
   if (hipMalloc(&c->popA, popb) != hipSuccess) return bail(DLBC_ERR_NOMEM);
   if (hipMalloc(&c->popB, popb) != hipSuccess) return bail(DLBC_ERR_NOMEM);
   if (hipMalloc(&c->S, sizeof(double) * components * c->geo.sp) != hipSuccess)
@@ -530,6 +593,25 @@ int dlbc_gpu_set_params(dlbc_gpu_ctx *c, const double *tau, const double *gcc, c
   p.sc = enable_shan_chen ? 1 : 0;
   p.has_solid = c->has_solid;
   c->p = p;
+  for (int f = 0; f < c->C; ++f) c->rho0[f] = rho0 ? rho0[f] : 1.0;
+  if (c->storage != DLBC_STORAGE_F64) {
+    ShiftParams sp{};
+    for (int f = 0; f < 2; ++f) {
+      sp.tau[f] = (float)p.tau[f];
+      sp.omega[f] = (float)p.omega[f];
+      sp.rho0[f] = (float)c->rho0[f];
+      sp.gwc[f] = (float)p.gwc[f];
+      for (int g = 0; g < 2; ++g) sp.gcc[f][g] = (float)p.gcc[f][g];
+      for (int k = 0; k < 3; ++k) sp.body[f][k] = (float)p.body[f][k];
+    }
+    for (int k = 0; k < 3; ++k) sp.gacc[k] = (float)p.gacc[k];
+    sp.eq_form = p.eq_form;
+    sp.psi_form = p.psi_form;
+    sp.sc = p.sc;
+    sp.has_solid = p.has_solid;
+    sp.has_body = p.has_body;
+    c->sparams = sp;
+  }
   c->params_set = true;
   return DLBC_OK;
 }
@@ -564,6 +646,7 @@ int dlbc_gpu_upload_mask(dlbc_gpu_ctx *c, const int32_t *host, int halo) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->has_solid = solid;
   c->p.has_solid = solid;
+  c->sparams.has_solid = solid;
   c->solid_rho_dirty = true;
   return DLBC_OK;
 }
@@ -588,8 +671,19 @@ int dlbc_gpu_upload_populations(dlbc_gpu_ctx *c, int f, const double *host, int 
       }
     HIPCHK(c, hipMemcpyAsync(c->scratch, stg.data(), sizeof(double) * o * q,
                              hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_aos_to_soa, dim3(blocks_for((int64_t)o, 256)), dim3(256), 0, c->stream,
-                       c->scratch, c->popA, c->popB, q, f, c->geo.sp, x0 * plane, (int)o);
+    const dim3 grid(blocks_for((int64_t)o, 256));
+    if (c->storage == DLBC_STORAGE_F64)
+      hipLaunchKernelGGL(k_aos_to_soa<double>, grid, dim3(256), 0, c->stream, c->scratch,
+                         static_cast<double *>(c->popA), static_cast<double *>(c->popB), q, f,
+                         c->geo.sp, x0 * plane, (int)o, 0.0);
+    else if (c->storage == DLBC_STORAGE_F32_SHIFT)
+      hipLaunchKernelGGL(k_aos_to_soa<float>, grid, dim3(256), 0, c->stream, c->scratch,
+                         static_cast<float *>(c->popA), static_cast<float *>(c->popB), q, f,
+                         c->geo.sp, x0 * plane, (int)o, c->rho0[f]);
+    else
+      hipLaunchKernelGGL(k_aos_to_soa<__half>, grid, dim3(256), 0, c->stream, c->scratch,
+                         static_cast<__half *>(c->popA), static_cast<__half *>(c->popB), q, f,
+                         c->geo.sp, x0 * plane, (int)o, c->rho0[f]);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
   }
@@ -609,8 +703,11 @@ int dlbc_gpu_download_populations(dlbc_gpu_ctx *c, int f, double *host, int halo
   for (int x0 = 0; x0 < c->n[0]; x0 += xchunk) {
     const int nxc = std::min(xchunk, c->n[0] - x0);
     const int cnt = nxc * plane;
-    hipLaunchKernelGGL(k_soa_to_aos, dim3(blocks_for(cnt, 256)), dim3(256), 0, c->stream,
-                       c->popA, c->scratch, q, f, c->geo.sp, x0 * plane, cnt);
+    with_loader(c, [&](auto, auto ld) {
+      hipLaunchKernelGGL((k_soa_to_aos<decltype(ld)>), dim3(blocks_for(cnt, 256)), dim3(256), 0,
+                         c->stream, ld, c->scratch, q, f, x0 * plane, cnt);
+      return 0;
+    });
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(stg.data(), c->scratch, sizeof(double) * cnt * q,
                              hipMemcpyDeviceToHost, c->stream));
@@ -691,12 +788,21 @@ int dlbc_gpu_synchronize(dlbc_gpu_ctx *c) {
 int dlbc_gpu_set_schedule(dlbc_gpu_ctx *c, int schedule) {
   if (!c || schedule < 0 || schedule > 2) return DLBC_ERR_ARG;
   if (schedule == DLBC_SCHED_FUSED && !(c->d == 3 && c->C == 2 && fused_d3q19_supported(c->geo)))
-    return fail(c, DLBC_ERR_UNSUPPORTED, "fused schedule needs D3Q19, 2 components and nz %% 64 == 0");
+    return fail(c, DLBC_ERR_UNSUPPORTED,
+                "fused schedule needs D3Q19, 2 components, n_y %% 8 == 0 and n_z %% 32 == 0");
+  if (schedule == DLBC_SCHED_TWO_PASS && c->storage != DLBC_STORAGE_F64)
+    return fail(c, DLBC_ERR_UNSUPPORTED, "the two-pass schedule needs FP64 storage");
   c->schedule = schedule;
   return DLBC_OK;
 }
 
 int dlbc_gpu_get_schedule(const dlbc_gpu_ctx *c) { return c ? effective_schedule(c) : -1; }
+
+int dlbc_gpu_set_launch(dlbc_gpu_ctx *c, int min_workgroups) {
+  if (!c || min_workgroups < 0 || min_workgroups > (1 << 20)) return DLBC_ERR_ARG;
+  c->target_blocks = min_workgroups;
+  return DLBC_OK;
+}
 
 int dlbc_gpu_density(dlbc_gpu_ctx *c, int f, double *host, int halo) {
   if (!c || !host || halo < 0 || f < 0 || f >= c->C) return DLBC_ERR_ARG;

@@ -297,23 +297,28 @@ inline bool fused_d3q19_supported(const Geo &g) {
   return g.nz % kFusedTZ == 0 && g.ny % kFusedTY == 0 && g.nx >= 4 && g.ny >= 4 && g.nz >= 4;
 }
 
-// Chunk length: one workgroup per CU (256 CUs; the register footprint allows
-// one 512-thread workgroup per CU) in a single round of blocks.
-inline void fused_d3q19_grid(const Geo &g, int &chunk, int &nchunks, int &ty, int &tz) {
+// Chunk length: x is cut into as many chunks as it takes for the grid to hold
+// at least `target` workgroups (one per CU by default: 256 CUs, and the FP64
+// register footprint allows one 512-thread workgroup per CU), each chunk at
+// least four planes long.
+inline void fused_d3q19_grid(const Geo &g, int target, int &chunk, int &nchunks, int &ty,
+                             int &tz) {
   ty = g.ny / kFusedTY;
   tz = g.nz / kFusedTZ;
   const int tiles = ty * tz;
-  int want = (256 + tiles - 1) / tiles;
+  if (target <= 0) target = 256;
+  int want = (target + tiles - 1) / tiles;
   if (want > g.nx / 4) want = g.nx / 4 > 0 ? g.nx / 4 : 1;
   chunk = (g.nx + want - 1) / want;
   nchunks = (g.nx + chunk - 1) / chunk;
 }
 
 inline int launch_fused_d3q19(const double *A, double *B, const uint32_t *nbw, double *S,
-                              const Geo &g, const StepParams &p, hipStream_t stream) {
+                              const Geo &g, const StepParams &p, int target_blocks,
+                              hipStream_t stream) {
   if (!fused_d3q19_supported(g)) return -1;
   int chunk, nchunks, ty, tz;
-  fused_d3q19_grid(g, chunk, nchunks, ty, tz);
+  fused_d3q19_grid(g, target_blocks, chunk, nchunks, ty, tz);
   const int nblocks = ty * tz * nchunks;
   constexpr int threads = 64 * kFusedTY;
   if (p.has_solid)

@@ -17,7 +17,7 @@ FLUID_INIT = {n: i for i, n in enumerate([
     "EqDistLamellaeFrac"])}
 MASK_INIT = {"None": 0, "File": 1, "Plates": 2, "Tube": 3, "Box": 4, "Walls": 5}
 AXIS = {"X": 0, "Y": 1, "Z": 2}
-STORAGE = {"f64": 0, "f32": 1, "f16": 2}
+STORAGE = {"f64": 0, "f32_shift": 1, "f16_shift": 2}
 SCHEDULES = {"auto": 0, "two_pass": 1, "fused": 2}
 ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "out of memory",
           -4: "unsupported configuration", -5: "call out of order"}
@@ -61,6 +61,7 @@ def lib():
         "dlbc_gpu_synchronize": (i, [vp]),
         "dlbc_gpu_set_schedule": (i, [vp, i]),
         "dlbc_gpu_get_schedule": (i, [vp]),
+        "dlbc_gpu_set_launch": (i, [vp, i]),
         "dlbc_gpu_download_populations": (i, [vp, i, _dp, i]),
         "dlbc_gpu_density": (i, [vp, i, _dp, i]),
         "dlbc_gpu_velocity": (i, [vp, i, _dp, i]),
@@ -248,6 +249,10 @@ class Lattice:
     def set_schedule(self, s):
         self._check(lib().dlbc_gpu_set_schedule(self.h, SCHEDULES[s] if isinstance(s, str) else s),
                     "set_schedule")
+
+    def set_launch(self, min_workgroups):
+        """Minimum workgroups per fused-kernel launch (0: default)."""
+        self._check(lib().dlbc_gpu_set_launch(self.h, int(min_workgroups)), "set_launch")
 
     @property
     def schedule(self):

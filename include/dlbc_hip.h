@@ -125,6 +125,11 @@ const char *dlbc_gpu_kernel_name(const dlbc_gpu_ctx *ctx, int i);
 int dlbc_gpu_synchronize(dlbc_gpu_ctx *ctx);
 int dlbc_gpu_set_schedule(dlbc_gpu_ctx *ctx, int schedule);
 int dlbc_gpu_get_schedule(const dlbc_gpu_ctx *ctx);
+/* Launch size of the fused kernels: the x axis is cut into chunks until a
+ * launch holds at least min_workgroups workgroups of 8 x 32 sites (0: the
+ * default, one workgroup per CU for FP64 storage and two for the shifted
+ * storages).  No counterpart in the reference (its loops are per rank). */
+int dlbc_gpu_set_launch(dlbc_gpu_ctx *ctx, int min_workgroups);
 
 /* Download component f into a host array [n0+2h]..[q]: physical sites are
  * written, halo sites are left untouched (the caller's exchangeHalo fills

@@ -1,0 +1,156 @@
+"""Shifted FP32 / FP16 population storage against the FP64 path.
+
+The shifted storages keep d_i = f_i - w_i rho0 in FP32 or FP16 and compute in
+FP32; they are an option of this library, not of the reference, so there is
+no bitwise oracle: each case runs the same seeded input through the FP64
+kernel (itself bit-identical to the oracle) and bounds the difference of the
+observables.  The bounds are stated per test; they are about twice the
+differences measured on MI355X.
+"""
+import numpy as np
+import pytest
+
+from cases import case
+
+pytestmark = pytest.mark.gpu
+
+W = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+
+
+@pytest.fixture(scope="module")
+def D():
+    import dlbc_amd
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("no GPU visible")
+    return dlbc_amd
+
+
+def _run(D, p, storage, steps, rho0=None, launch=None):
+    pp = dict(p)
+    if rho0 is not None:
+        pp["rho0"] = rho0
+    L = D.Lattice.from_params(pp, storage=storage, schedule="fused")
+    if launch is not None:
+        L.set_launch(launch)
+    m0 = [L.total_mass(f) for f in range(2)]
+    L.step(steps)
+    return L, m0
+
+
+def _diff(L, ref, m0):
+    out = {}
+    for f in range(2):
+        out["drho%d" % f] = float(np.abs(L.density(f) - ref.density(f)).max())
+        out["du%d" % f] = float(np.abs(L.velocity(f) - ref.velocity(f)).max())
+        out["dmass%d" % f] = abs(L.total_mass(f) - m0[f]) / m0[f]
+    out["vol"] = int((L.colour() > 0).sum())
+    out["vol_ref"] = int((ref.colour() > 0).sum())
+    return out
+
+
+# bounds per storage: max |d rho|, max |d u|, relative mass drift, droplet volume
+BOUNDS = {"f32_shift": (3e-5, 1e-5, 1e-4, 0.0), "f16_shift": (2.5e-2, 1.3e-2, 3e-3, 0.02)}
+
+
+@pytest.mark.parametrize("storage", ["f32_shift", "f16_shift"])
+@pytest.mark.parametrize("name", ["laplace_droplet_3d", "merging_droplets_3d"])
+def test_droplets_1000_steps(D, name, storage):
+    """The reference's 64^3 BDist2 Shan-Chen droplet cases, 1000 steps:
+    density and velocity fields, mass drift and droplet volume (sites of
+    positive colour) against FP64.  FP32 drifts by up to 5e-5 in mass over
+    1000 steps: in the steady bulk phases every site repeats the same
+    rounding of its stored deviations each step."""
+    p = case(name)
+    ref, _ = _run(D, p, "f64", 1000)
+    L, m0 = _run(D, p, storage, 1000, rho0=[0.37, 0.37])
+    r = _diff(L, ref, m0)
+    drho, du, dm, dvol = BOUNDS[storage]
+    assert max(r["drho0"], r["drho1"]) <= drho, r
+    assert max(r["du0"], r["du1"]) <= du, r
+    assert max(r["dmass0"], r["dmass1"]) <= dm, r
+    assert abs(r["vol"] - r["vol_ref"]) <= dvol * r["vol_ref"], r
+
+
+@pytest.mark.parametrize("storage,drho", [("f32_shift", 2e-7), ("f16_shift", 1.5e-6)])
+def test_spinodal_short(D, storage, drho):
+    """Spinodal decomposition (0.7 +- 0.01, SecondOrder) over 100 steps, while
+    the instability is still growing slowly enough to compare fields."""
+    p = dict(d=3, q=19, gn=[64, 64, 64], components=2, tau=[1.0, 1.0],
+             eqDistForm="SecondOrder", gcc=[[0.0, 3.6], [3.6, 0.0]], enableShanChen=True,
+             psiForm="Exponential", fluidInit=["EqDistPerturb"] * 2,
+             fluidDensities=[[0.7], [0.7]], fluidPerturb=[0.01, 0.01],
+             randomReseed=[True, True], randomSeed=[42, 43])
+    ref, _ = _run(D, p, "f64", 100)
+    L, m0 = _run(D, p, storage, 100, rho0=[0.7, 0.7])
+    r = _diff(L, ref, m0)
+    assert max(r["drho0"], r["drho1"]) <= drho, r
+
+
+def _channel(n):
+    import bench
+    p = bench.channel_params(n)
+    p["maskArray"] = bench.channel_mask(n)
+    return p
+
+
+@pytest.mark.parametrize("storage,drho", [("f32_shift", 1e-4), ("f16_shift", 5e-2)])
+def test_channel_walls_body_force(D, storage, drho):
+    """The benchmark's channel (walls, random spheres, body force) at 64^3,
+    300 steps: finite and close to FP64."""
+    p = _channel(64)
+    ref, _ = _run(D, p, "f64", 300)
+    L, m0 = _run(D, p, storage, 300, rho0=[0.525, 0.525])
+    for f in range(2):
+        assert np.isfinite(ref.density(f)).all()
+    r = _diff(L, ref, m0)
+    assert max(r["drho0"], r["drho1"]) <= drho, r
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32_shift", "f16_shift"])
+def test_launch_size_does_not_change_results(D, storage):
+    """Cutting x into more chunks changes only the schedule: bitwise equal
+    populations for 256, 512 and 2048 workgroups (64 x 128 x 128 lattice)."""
+    p = _channel(64)
+    p["gn"] = [64, 128, 128]
+    mask = np.zeros(p["gn"], np.int32)
+    mask[:, :, 0] = 1
+    mask[:, :, -1] = 1
+    mask[20:30, 40:60, 50:70] = 1
+    p["maskArray"] = mask
+    pops = []
+    for launch in (256, 512, 2048):
+        L, _ = _run(D, p, storage, 20, rho0=[0.37, 0.37], launch=launch)
+        pops.append([L.population(f) for f in range(2)])
+    for other in pops[1:]:
+        for f in range(2):
+            assert np.array_equal(other[f].view(np.uint64), pops[0][f].view(np.uint64))
+
+
+@pytest.mark.parametrize("storage,dt", [("f32_shift", np.float32), ("f16_shift", np.float16)])
+def test_upload_download_rounding(D, storage, dt):
+    """Upload stores RN(f_i - w_i rho0) (FP64 -> FP32 [-> FP16]); download
+    returns w_i rho0 + the stored value in FP64."""
+    n = [8, 8, 32]
+    rng = np.random.default_rng(11)
+    rho0 = [0.5, 0.25]
+    L = D.Lattice(3, 19, 2, n, storage=storage)
+    L.set_params([1.0, 1.0], rho0=rho0)
+    L.upload_mask(np.zeros(n, np.int32))
+    for f in range(2):
+        pop = rng.uniform(0.0, 1.0, tuple(n) + (19,)) * W
+        L.upload_population(f, pop)
+        dev = (pop - W * rho0[f]).astype(np.float32).astype(dt)
+        exp = W * rho0[f] + dev.astype(np.float64)
+        got = L.population(f)
+        assert np.array_equal(got.view(np.uint64), exp.view(np.uint64))
+
+
+def test_shifted_refuses_fp64_only_paths(D):
+    L = D.Lattice(3, 19, 2, [8, 8, 32], storage="f32_shift")
+    L.set_params([1.0, 1.0])
+    with pytest.raises(D.DlbcError):
+        L.set_schedule("two_pass")
+    L.upload_mask(np.zeros([8, 8, 32], np.int32))
+    with pytest.raises(D.DlbcError):
+        L.advect()

@@ -117,3 +117,14 @@ def test_exact_constant_division_selftest():
     it equals IEEE division bit for bit."""
     import dlbc_amd as D
     assert D.lib().dlbc_selftest_division(4_000_000, 2026) == 0
+
+
+@pytest.mark.parametrize("d,q,nc,n", [(2, 9, 2, [64, 64]), (3, 19, 1, [8, 8, 32]),
+                                      (3, 19, 2, [8, 12, 32]), (3, 19, 2, [8, 8, 48])])
+def test_shifted_storage_shape_checks(d, q, nc, n):
+    """The shifted storages exist for the fused D3Q19 two-component kernel
+    only (n_y % 8 == 0, n_z % 32 == 0); other shapes are refused before any
+    device call."""
+    import dlbc_amd as D
+    with pytest.raises(D.DlbcError, match="unsupported"):
+        D.Lattice(d, q, nc, n, storage="f32_shift")
