@@ -83,6 +83,23 @@ def build_inputs(D, p, mask):
     return pops
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "step_kernels_channel256_traffic.txt")
+
+
+def pmc_traffic(storage, n):
+    """HBM bytes per launch of the step kernel on this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters scaled by a
+    copy kernel of known size); None for sizes that summary does not cover."""
+    if n != 256 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as fh:
+        for line in fh:
+            parts = line.split()
+            if len(parts) >= 6 and parts[0] == "traffic" and parts[1] == storage:
+                return int(parts[5])
+    return None
+
+
 def make_lattice(D, p, mask, pops, schedule, storage="f64", rho0=None):
     L = D.Lattice(3, Q, 2, p["gn"], storage=storage)
     L.set_params(p["tau"], p["gcc"], None, None, p["bodyForce"], p["eqDistForm"], p["psiForm"],
@@ -128,7 +145,8 @@ def storage_leg(D, p, mask, pops, storage, steps, warmup, fluid):
            "ms_per_step": round(1e3 * dt / steps, 4), "kernel": dom,
            "launch_ms": round(kern[dom], 4), "bytes_per_launch": b,
            "achieved": round(b / (kern[dom] * 1e-3) / 1e9, 1),
-           "frac": round(b / (kern[dom] * 1e-3) / HBM_PEAK, 4), "rho0": rho0,
+           "frac": round(b / (kern[dom] * 1e-3) / HBM_PEAK, 4),
+           "traffic": pmc_traffic(storage, p["gn"][0]), "rho0": rho0,
            "mass_rel_drift": float("%.3g" % drift)}
     del L
     return out
@@ -239,7 +257,7 @@ def main():
                    "mass_rel_drift": float("%.3g" % drift)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 1),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK, 4), "traffic": pmc_traffic("f64", n),
                      "bytes_per_launch": b_alg, "launch_ms": round(kern[dom], 4),
                      "step_frac": round(mlups * 1e6 * (b_alg / sites) / HBM_PEAK, 4)},
     }

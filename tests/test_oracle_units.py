@@ -235,3 +235,44 @@ def test_bounce_back_wall_no_flux(oracle_mod):
     assert post[1, 1, 1] == pre[1, 1, bounce[1]]
     o.run(30)
     assert abs(o.total_mass(0) - m0) < 1e-12 * m0
+
+
+def _wall_box(oracle_mod, gcc, gwc, rho=0.6):
+    """Uniform fluid at rest between solid planes z = 0 and z = n-1."""
+    n = [5, 4, 7]
+    mask = np.zeros(n, np.int32)
+    mask[:, :, 0] = 1
+    mask[:, :, -1] = 1
+    p = dict(d=3, q=19, gn=n, components=2, tau=[1.0, 1.0], eqDistForm="SecondOrder",
+             gcc=gcc, gwc=gwc, enableShanChen=True, psiForm="Exponential",
+             fluidInit=["EqDist"] * 2, fluidDensities=[[rho], [rho]], maskArray=mask)
+    o = oracle_mod.Oracle.from_params(p)
+    o.run(1)
+    phys = (slice(2, -2),) * 3
+    return [o.raw("force", f)[phys].copy() for f in range(2)], oracle_mod.psi_value(1, rho)
+
+
+def test_shan_chen_force_solid_substitution(oracle_mod):
+    """lb/force.d:253-283: a solid neighbour contributes the site's own psi.
+    On a uniform state next to walls every force is then exactly zero; any
+    other value at the solid sites (their density 1, or 0) leaves a force on
+    the layers next to the walls."""
+    F, _ = _wall_box(oracle_mod, [[0.0, 3.6], [3.6, 0.0]], [0.0, 0.0])
+    for f in range(2):
+        assert np.all(F[f][:, :, 1:-1] == 0.0)
+
+
+def test_shan_chen_wall_force_closed_form(oracle_mod):
+    """lb/force.d:284-307 with gcc = 0: the site next to the wall z = 0 has
+    solid neighbours along c_z = -1 (one with w = 1/18, four with w = 1/36)
+    whose density is 1 (solid sites hold [1, 0, ...], fields/init.d:469-477),
+    so F = -sum_i w_i psi gwc c_i = (0, 0, +psi gwc / 6); the mirror image
+    at z = n-2; zero in between."""
+    gwc = [0.25, -0.4]
+    F, psi = _wall_box(oracle_mod, [[0.0, 0.0], [0.0, 0.0]], gwc)
+    for f in range(2):
+        fz = psi * gwc[f] / 6.0
+        assert np.allclose(F[f][:, :, 1, 2], fz, rtol=4e-16, atol=0)
+        assert np.allclose(F[f][:, :, -2, 2], -fz, rtol=4e-16, atol=0)
+        assert np.all(np.abs(F[f][:, :, 1:-1, :2]) <= 1e-18)
+        assert np.all(F[f][:, :, 2:-2, :] == 0.0)
