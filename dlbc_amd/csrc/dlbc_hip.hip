@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -207,6 +208,25 @@ __global__ void k_field(Ld ld, const uint8_t *__restrict__ mask, double *__restr
       out[s] = density<L>(n) - density<L>(m);
     }
   }
+}
+
+// Densities of both components and the fluid flag of the sites of a box
+// [x0, x0+ex) x [y0, y0+ey) x [z0, z0+ez), row-major:
+// out[k * (C+1) + f] = rho_f, out[k * (C+1) + C] = 1 (fluid) or 0 (solid).
+template <class L, class Ld>
+__global__ void k_box_density(Ld ld, const uint8_t *__restrict__ mask, double *__restrict__ out,
+                              Geo g, int x0, int y0, int z0, int ex, int ey, int ez, int C,
+                              int has_solid) {
+  const int t = blockIdx.x * 64 + threadIdx.x;
+  if (t >= ex * ey * ez) return;
+  const int s = site_of(x0 + t / (ey * ez), y0 + (t / ez) % ey, z0 + t % ez, g);
+  for (int f = 0; f < C; ++f) {
+    double n[L::q];
+#pragma unroll
+    for (int i = 0; i < L::q; ++i) n[i] = ld(f, i, s);
+    out[(int64_t)t * (C + 1) + f] = density<L>(n);
+  }
+  out[(int64_t)t * (C + 1) + C] = (has_solid && mask[s]) ? 0.0 : 1.0;
 }
 
 // Deterministic double-double reduction over fluid sites.  Stage 1: every
@@ -497,6 +517,93 @@ int reduce_launch(dlbc_gpu_ctx *c, int f, int what, double *out2) {
   HIPCHK(c, hipMemcpyAsync(out2, res, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return DLBC_OK;
+}
+
+// volumeAveragedDensity (src/dlbc/lb/laplace.d:236-281) with volumeAverage 1:
+// mean density per component over the fluid sites of the 3^d cube centred at
+// `centre` (clipped at the lattice edge, as the reference's cube does not
+// wrap), summed in row-major order.
+int cube_density(dlbc_gpu_ctx *c, const int *centre, double *avg) {
+  int lo[3] = {0, 0, 0}, ext[3] = {1, 1, 1};
+  for (int vd = 0; vd < c->d; ++vd) {
+    lo[vd] = std::max(0, centre[vd] - 1);
+    ext[vd] = std::min(c->n[vd] - 1, centre[vd] + 1) - lo[vd] + 1;
+  }
+  const int cnt = ext[0] * ext[1] * ext[2];
+  const int C = c->C;
+  int rc = ensure_scratch(c, sizeof(double) * cnt * (C + 1));
+  if (rc) return rc;
+  with_loader(c, [&](auto lat, auto ld) {
+    hipLaunchKernelGGL((k_box_density<decltype(lat), decltype(ld)>), dim3(1), dim3(64), 0,
+                       c->stream, ld, c->mask, c->scratch, c->geo, lo[0], lo[1], lo[2], ext[0],
+                       ext[1], ext[2], C, c->has_solid);
+    return 0;
+  });
+  HIPCHK(c, hipGetLastError());
+  std::vector<double> buf((size_t)cnt * (C + 1));
+  HIPCHK(c, hipMemcpyAsync(buf.data(), c->scratch, buf.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int f = 0; f < C; ++f) {
+    double sum = 0.0;
+    int n = 0;
+    for (int k = 0; k < cnt; ++k)
+      if (buf[(size_t)k * (C + 1) + C] != 0.0) {
+        sum += buf[(size_t)k * (C + 1) + f];
+        ++n;
+      }
+    avg[f] = sum / n;
+  }
+  return DLBC_OK;
+}
+
+// pressurePsi (src/dlbc/lb/density.d:316-326).
+double sc_pressure(const dlbc_gpu_ctx *c, const double *den) {
+  auto psi = [&](double r) { return c->p.psi_form == 0 ? r : psi_exponential(r); };
+  double pr = 0.0;
+  for (int i = 0; i < c->C; ++i) {
+    pr += den[i];
+    for (int j = 0; j < c->C; ++j) pr += 0.5 * c->p.gcc[i][j] * psi(den[i]) * psi(den[j]);
+  }
+  return pr * kCss;
+}
+
+// Droplet deformation of a 2-D lattice (src/dlbc/lb/laplace.d:118-199):
+// inertia tensor of the sites with rho_0 > 0.35 about their centre of mass.
+double deformation_2d(const dlbc_gpu_ctx *c, const std::vector<double> &rho,
+                      const std::vector<uint8_t> &solid) {
+  const double cutoff = 0.35;
+  const int nx = c->n[0], ny = c->n[1];
+  double rx = 0.0, ry = 0.0, mass = 0.0;
+  for (int x = 0; x < nx; ++x)
+    for (int y = 0; y < ny; ++y) {
+      const double den = rho[(size_t)x * ny + y];
+      if (!solid[(size_t)x * ny + y] && den > cutoff) {
+        mass += den;
+        rx += den * x;
+        ry += den * y;
+      }
+    }
+  rx /= mass;
+  ry /= mass;
+  double ixx = 0.0, ixy = 0.0, iyy = 0.0;
+  for (int x = 0; x < nx; ++x)
+    for (int y = 0; y < ny; ++y) {
+      const double den = rho[(size_t)x * ny + y];
+      if (!solid[(size_t)x * ny + y] && den > cutoff) {
+        const double xx = x - rx, yy = y - ry;
+        ixx += den * (yy * yy);
+        ixy += den * (xx * yy);
+        iyy += den * (xx * xx);
+      }
+    }
+  const double tr = ixx + iyy, det = ixx * iyy - ixy * ixy;
+  const double ev1 = 0.5 * tr + std::sqrt(0.25 * tr * tr - det);
+  const double ev2 = 0.5 * tr - std::sqrt(0.25 * tr * tr - det);
+  const double l = std::sqrt(5.0 * std::max(ev1, ev2) / mass);
+  const double b = std::sqrt(5.0 * std::min(ev1, ev2) / mass);
+  const double D = (l - b) / (l + b);
+  return std::isnan(D) ? 0.0 : D;
 }
 
 }  // namespace
@@ -843,6 +950,61 @@ int dlbc_gpu_total_momentum(dlbc_gpu_ctx *c, int f, double *out) {
     out[j] = r[0];
     out[c->d + j] = r[1];
   }
+  return DLBC_OK;
+}
+
+int dlbc_gpu_laplace(dlbc_gpu_ctx *c, double *out) {
+  if (!c || !out) return DLBC_ERR_ARG;
+  if (c->C != 2 || !c->p.sc)
+    return fail(c, DLBC_ERR_UNSUPPORTED, "Laplace diagnostics need two Shan-Chen components");
+  if (!c->params_set || !c->pops_uploaded[0] || !c->pops_uploaded[1])
+    return fail(c, DLBC_ERR_STATE, "set parameters and upload populations first");
+  int cin[3], cout[3];
+  for (int vd = 0; vd < c->d; ++vd) {
+    cin[vd] = (int)(c->n[vd] * 0.5);  // laplace.d:73-76
+    cout[vd] = 1;
+  }
+  double din[kMaxC], dout[kMaxC];
+  int rc = cube_density(c, cin, din);
+  if (!rc) rc = cube_density(c, cout, dout);
+  if (rc) return rc;
+  const double pin = sc_pressure(c, din), pout = sc_pressure(c, dout);
+  double mass[2];
+  rc = reduce_launch(c, 0, 0, mass);
+  if (rc) return rc;
+  const double gmass = mass[0] + mass[1];
+  const double eff = gmass - (double)c->geo.sites * dout[0];  // laplace.d:95
+  const double kPi = 3.14159265358979323846;
+  double r, sigma;
+  if (c->d == 3) {
+    r = std::pow(eff / (4.0 / 3.0 * kPi * (din[0] - dout[0])), 1.0 / 3.0);
+    sigma = r * (pin - pout) / 2.0;
+  } else if (c->d == 2) {
+    r = std::pow(eff / (kPi * (din[0] - dout[0])), 1.0 / 2.0);
+    sigma = r * (pin - pout);
+  } else {
+    r = eff / (din[0] - dout[0]);
+    sigma = r * (pin - pout);
+  }
+  double D = std::nan("");
+  if (c->d == 2) {
+    rc = ensure_scratch(c, sizeof(double) * c->geo.sites);
+    if (!rc) rc = field_launch(c, c->scratch, 0, 0, 0);
+    if (rc) return rc;
+    std::vector<double> rho(c->geo.sites);
+    std::vector<uint8_t> solid(c->geo.sites);
+    HIPCHK(c, hipMemcpyAsync(rho.data(), c->scratch, sizeof(double) * c->geo.sites,
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(solid.data(), c->mask, c->geo.sites, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    D = deformation_2d(c, rho, solid);
+  }
+  out[0] = sigma;
+  out[1] = r;
+  out[2] = pin;
+  out[3] = pout;
+  out[4] = D;
   return DLBC_OK;
 }
 

@@ -62,6 +62,7 @@ def lib():
         "dlbc_gpu_set_schedule": (i, [vp, i]),
         "dlbc_gpu_get_schedule": (i, [vp]),
         "dlbc_gpu_set_launch": (i, [vp, i]),
+        "dlbc_gpu_laplace": (i, [vp, _dp]),
         "dlbc_gpu_download_populations": (i, [vp, i, _dp, i]),
         "dlbc_gpu_density": (i, [vp, i, _dp, i]),
         "dlbc_gpu_velocity": (i, [vp, i, _dp, i]),
@@ -295,6 +296,14 @@ class Lattice:
         y = np.zeros_like(x)
         self._check(lib().dlbc_gpu_eval_psi(self.h, int(form), x, y, x.size), "eval_psi")
         return y
+
+    def laplace(self):
+        """Laplace diagnostics (src/dlbc/lb/laplace.d:55-218) of a centred
+        two-component droplet: sigma, radius, in/out pressure, deformation."""
+        out = np.zeros(5)
+        self._check(lib().dlbc_gpu_laplace(self.h, out), "laplace")
+        return dict(zip(("sigma", "radius", "in_pressure", "out_pressure", "deformation"),
+                        out.tolist()))
 
     def device_bytes(self):
         return lib().dlbc_gpu_device_bytes(self.h)

@@ -151,6 +151,14 @@ int dlbc_gpu_colour(dlbc_gpu_ctx *ctx, int f1, int f2, double *host, int halo);
 int dlbc_gpu_total_mass(dlbc_gpu_ctx *ctx, int f, double *out /*2*/);
 int dlbc_gpu_total_momentum(dlbc_gpu_ctx *ctx, int f, double *out /*2d*/);
 
+/* Laplace diagnostics of a centred two-component droplet, as dumpLaplace
+ * (src/dlbc/lb/laplace.d:55-218): out[0] sigma, out[1] measured radius,
+ * out[2] pressure inside, out[3] pressure outside (volume-averaged densities
+ * of the 3^d cubes at int(n/2) and at 1, pressurePsi of lb/density.d:316-326),
+ * out[4] droplet deformation for d = 2 (NaN otherwise).  The reference's
+ * relative change of sigma is left to the caller. */
+int dlbc_gpu_laplace(dlbc_gpu_ctx *ctx, double *out /*5*/);
+
 /* Evaluate psi (form) on n densities on the device (parity tests of the
  * correctly rounded exp).  x and y are host arrays. */
 int dlbc_gpu_eval_psi(dlbc_gpu_ctx *ctx, int psi_form, const double *x, double *y, int64_t n);

@@ -256,6 +256,96 @@ class Oracle:
         lib().or_local_total_momentum(self.h, f, out)
         return out
 
+    # -- diagnostics ---------------------------------------------------------
+    def laplace(self, gcc, psi_form):
+        """dumpLaplace (src/dlbc/lb/laplace.d:55-218) for two components:
+        volume-averaged densities of the 3^d cubes at int(gn/2) and at 1
+        (volumeAveragedDensity, laplace.d:236-281), their pressures
+        (pressurePsi, lb/density.d:316-326), the radius from the excess mass of
+        component 0, sigma from Laplace's law, and for d = 2 the deformation
+        from the inertia tensor of the sites with rho_0 > 0.35 (d != 2: NaN)."""
+        import math
+        d, n = self.d, self.n
+        dens = [self.density(f) for f in range(2)]
+        mask = self.mask()
+        if isinstance(psi_form, str):
+            psi_form = PSI_FORMS[psi_form]
+        css = 1.0 / 3.0
+
+        def cube(offset):
+            lo = [max(0, o - 1) for o in offset]
+            hi = [min(n[vd] - 1, offset[vd] + 1) for vd in range(d)]
+            out = []
+            for f in range(2):
+                tot, cnt = 0.0, 0
+                for idx in np.ndindex(*[hi[vd] - lo[vd] + 1 for vd in range(d)]):
+                    site = tuple(lo[vd] + idx[vd] for vd in range(d))
+                    if mask[site] == 0:
+                        tot += float(dens[f][site])
+                        cnt += 1
+                out.append(tot / cnt)
+            return out
+
+        def pressure(den):
+            pr = 0.0
+            for i in range(2):
+                pr += den[i]
+                for j in range(2):
+                    pr += 0.5 * gcc[i][j] * psi_value(psi_form, den[i]) * \
+                        psi_value(psi_form, den[j])
+            return pr * css
+
+        in_den = cube([int(v * 0.5) for v in n])
+        out_den = cube([1] * d)
+        in_p, out_p = pressure(in_den), pressure(out_den)
+        eff = self.total_mass(0) - float(np.prod(n)) * out_den[0]
+        if d == 3:
+            r = math.pow(eff / (4.0 / 3.0 * math.pi * (in_den[0] - out_den[0])), 1.0 / 3.0)
+            sigma = r * (in_p - out_p) / 2.0
+        elif d == 2:
+            r = math.pow(eff / (math.pi * (in_den[0] - out_den[0])), 1.0 / 2.0)
+            sigma = r * (in_p - out_p)
+        else:
+            r = eff / (in_den[0] - out_den[0])
+            sigma = r * (in_p - out_p)
+        deform = float("nan")
+        if d == 2:  # laplace.d:118-199
+            cutoff = 0.35
+            rx = ry = mass = 0.0
+            for x in range(n[0]):
+                for y in range(n[1]):
+                    den = float(dens[0][x, y])
+                    if mask[x, y] == 0 and den > cutoff:
+                        mass += den
+                        rx += den * x
+                        ry += den * y
+            rx /= mass
+            ry /= mass
+            ixx = ixy = iyy = 0.0
+            for x in range(n[0]):
+                for y in range(n[1]):
+                    den = float(dens[0][x, y])
+                    if mask[x, y] == 0 and den > cutoff:
+                        xx, yy = x - rx, y - ry
+                        ixx += den * (yy * yy)
+                        ixy += den * (xx * yy)
+                        iyy += den * (xx * xx)
+            tr = ixx + iyy
+            det = ixx * iyy - ixy * ixy
+            disc = 0.25 * tr * tr - det
+            if disc < 0.0:
+                deform = 0.0
+            else:
+                ev1 = 0.5 * tr + math.sqrt(disc)
+                ev2 = 0.5 * tr - math.sqrt(disc)
+                lmaj = math.sqrt(5.0 * max(ev1, ev2) / mass)
+                bmin = math.sqrt(5.0 * min(ev1, ev2) / mass)
+                deform = (lmaj - bmin) / (lmaj + bmin)
+                if math.isnan(deform):
+                    deform = 0.0
+        return {"sigma": sigma, "radius": r, "in_pressure": in_p, "out_pressure": out_p,
+                "deformation": deform}
+
     # -- initialisation ----------------------------------------------------
     def init_mask(self, kind="None", axis=0, wall_offset=0, dim_has_wall=None,
                   wall_thickness=1):

@@ -228,6 +228,27 @@ def main():
           "params": "lb/init-2d/fluid-init-eqdistperturb-2d.in",
           "layout": "np=4, nc=[2,2], shiftSeedByRank=false: one 32x32 quadrant kept",
           "tolerance": {"t0": 0.0}})
+    # Laplace diagnostics written every 100 steps (lb/laplace.d:55-281), printed
+    # with 7 significant digits: t, sigma, measuredR, inPres, outPres, D.
+    for dim in ("2d", "3d"):
+        d = os.path.join(REF, "lb/laplace-droplet-%s/reference-data" % dim)
+        path = find(d, r"laplace-laplace-droplet-%s-.*\.asc" % dim)
+        rows = []
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("#") or not line.strip():
+                    continue
+                v = line.split()
+                rows.append([float(v[2]), float(v[3]), float(v[5]), float(v[6]), float(v[7]),
+                             float(v[8])])
+        a = np.array(rows)
+        _, first = np.unique(a[:, 0], return_index=True)  # the final line repeats t = 10000
+        save("laplace_trace_%s" % dim, {"trace": a[np.sort(first)]},
+             {"source": "tests/runnable/tests/lb/laplace-droplet-%s/reference-data/%s"
+                        % (dim, os.path.basename(path)),
+              "columns": ["t", "sigma", "measuredR", "inPres", "outPres", "D"],
+              "precision": "7 significant digits (%e)",
+              "params": "lb/laplace-droplet-%s/laplace-droplet-%s.in" % (dim, dim)})
     with open(os.path.join(OUT, "manifest.json"), "w") as fh:
         json.dump(MANIFEST, fh, indent=1, sort_keys=True)
 

@@ -326,3 +326,43 @@ def test_fused_equals_two_pass_with_body_force(D):
     b.step(60)
     for f in range(2):
         assert_close(a.population(f), b.population(f), 0.0, "fused vs two-pass f%d" % f)
+
+
+def _trace_close(got, row, rel=6e-7):
+    t, sigma, radius, p_in, p_out, deform = row
+    for name, ref in (("sigma", sigma), ("radius", radius), ("in_pressure", p_in),
+                      ("out_pressure", p_out)):
+        assert abs(got[name] - ref) <= rel * abs(ref) + 1e-12, (t, name, got[name], ref)
+    if np.isnan(deform):
+        assert np.isnan(got["deformation"])
+    else:  # round-off sized on both sides (a circular droplet)
+        assert abs(got["deformation"]) <= 2e-8 and abs(deform) <= 1e-8, (t, got, deform)
+
+
+@pytest.mark.parametrize("dim", ["2d", "3d"])
+def test_laplace_trace_against_reference(D, dim):
+    """The reference's Laplace output (sigma, radius, pressures, deformation;
+    every 100 steps to t = 10000, 7 significant digits) reproduced by the
+    product's step and its device-side Laplace diagnostics."""
+    import os
+    from cases import GOLDEN
+    g = np.load(os.path.join(GOLDEN, "laplace_trace_%s.npz" % dim))["trace"]
+    L = D.Lattice.from_params(case("laplace_droplet_%s" % dim))
+    t = 0
+    for row in g:
+        L.step(int(row[0]) - t)
+        t = int(row[0])
+        _trace_close(L.laplace(), row)
+
+
+def test_laplace_equals_oracle(D, oracle_mod):
+    """Device Laplace diagnostics == the oracle's restatement on the same
+    (bitwise identical) state; only the total-mass summation differs
+    (double-double on the device, sequential in the oracle)."""
+    p = case("laplace_droplet_3d", gn=[32, 32, 32])
+    g, o = _pair(D, oracle_mod, p)
+    g.step(50)
+    o.run(50)
+    a, b = g.laplace(), o.laplace(p["gcc"], p["psiForm"])
+    for k in ("sigma", "radius", "in_pressure", "out_pressure"):
+        assert abs(a[k] - b[k]) <= 1e-13 * abs(b[k]), (k, a[k], b[k])

@@ -6,11 +6,12 @@ bit-for-bit.  Two fixtures were written by older code revisions than the
 sources the oracle restates; their looser bounds are stated where used.
 """
 import math
+import os
 
 import numpy as np
 import pytest
 
-from cases import assert_close, case, expand, golden, tolerance
+from cases import GOLDEN, assert_close, case, expand, golden, tolerance
 
 COLOURS = ("red", "blue")
 
@@ -200,3 +201,42 @@ def test_poiseuille_2d_secondorder(oracle_mod):
         assert np.ptp(off) < 1e-12, "not the Poiseuille parabola"
         assert abs(off[0]) < 2e-4  # a constant of the order of the acceleration
         assert np.all(o.velocity(f)[[0, 31]] == 0.0)  # zero on solid sites
+
+
+def _trace_row_close(got, row, rel=6e-7):
+    """One line of the reference's Laplace output (7 significant digits)."""
+    t, sigma, radius, p_in, p_out, deform = row
+    for name, ref in (("sigma", sigma), ("radius", radius), ("in_pressure", p_in),
+                      ("out_pressure", p_out)):
+        assert abs(got[name] - ref) <= rel * abs(ref) + 1e-12, (t, name, got[name], ref)
+    if np.isnan(deform):
+        assert np.isnan(got["deformation"]), (t, got["deformation"])
+    else:  # round-off sized on both sides (a circular droplet)
+        assert abs(got["deformation"]) <= 2e-8 and abs(deform) <= 1e-8, (t, got, deform)
+
+
+def test_laplace_droplet_2d_trace(oracle_mod):
+    """laplace-droplet-2d: sigma, radius, pressures and deformation every 100
+    steps up to t = 10000 (lb/laplace.d:55-218) against the reference's
+    .asc output."""
+    g = np.load(os.path.join(GOLDEN, "laplace_trace_2d.npz"))["trace"]
+    p = case("laplace_droplet_2d")
+    o = oracle_mod.Oracle.from_params(p)
+    t = 0
+    for row in g:
+        o.run(int(row[0]) - t)
+        t = int(row[0])
+        _trace_row_close(o.laplace(p["gcc"], p["psiForm"]), row)
+
+
+def test_laplace_droplet_3d_trace_start(oracle_mod):
+    """laplace-droplet-3d (64^3): the first four lines of the reference's
+    Laplace output (t = 0 .. 300); the GPU test follows all 101."""
+    g = np.load(os.path.join(GOLDEN, "laplace_trace_3d.npz"))["trace"]
+    p = case("laplace_droplet_3d")
+    o = oracle_mod.Oracle.from_params(p)
+    t = 0
+    for row in g[:4]:
+        o.run(int(row[0]) - t)
+        t = int(row[0])
+        _trace_row_close(o.laplace(p["gcc"], p["psiForm"]), row)
