@@ -175,6 +175,133 @@ def cpu_baseline(p, mask, pops, steps=2):
                       "plugin hook, is left out of this leg" % (steps, "x".join(map(str, n)))}
 
 
+def droplet_params(n, radius):
+    """Centred droplet: red 1.0 inside / 0.05 outside, blue swapped (sharp
+    interface), periodic, tau 1, g 3.6, psi = 1 - exp(-rho), BDist2."""
+    return dict(d=3, q=19, gn=[n, n, n], components=2, tau=[1.0, 1.0], eqDistForm="BDist2",
+                gcc=[[0.0, 3.6], [3.6, 0.0]], enableShanChen=True, psiForm="Exponential",
+                fluidInit=["EqDistSphere"] * 2, fluidDensities=[[1.0, 0.05], [0.05, 1.0]],
+                initRadius=float(radius))
+
+
+def droplet_workload(D, n, steps, storages=("f16_shift", "f32_shift")):
+    """Shifted storages against FP64 on the droplet: throughput of each, and
+    after `steps` steps the largest density and velocity differences, the
+    droplet radius of the Laplace diagnostics and the mass drift."""
+    import torch
+    radius = 96.0 * n / 512.0
+    p = droplet_params(n, radius)
+    t0 = time.perf_counter()
+    mask = np.zeros((n, n, n), np.int32)
+    pops = []
+    for f in range(2):
+        pop = D.init_fluid(3, Q, p["gn"], "EqDistSphere", p["fluidDensities"][f], "BDist2", 0.0,
+                           None, radius, None, None, 0, 0.0, None, halo=0)
+        pops.append(pop)
+    log("droplet inputs %.1f s" % (time.perf_counter() - t0))
+
+    def run(storage):
+        L = D.Lattice(3, Q, 2, p["gn"], storage=storage)
+        L.set_params(p["tau"], p["gcc"], None, None, None, p["eqDistForm"], p["psiForm"], True,
+                     [0.525, 0.525])
+        L.upload_mask(mask)
+        for f in range(2):
+            L.upload_population(f, pops[f])
+        m0 = [L.total_mass(f) for f in range(2)]
+        lap0 = L.laplace()
+        L.synchronize()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        L.step(steps)
+        L.synchronize()
+        dt = time.perf_counter() - t
+        out = {"mlups": round(n ** 3 * steps / dt / 1e6, 1), "seconds": round(dt, 2),
+               "radius_t0": lap0["radius"], "laplace": L.laplace(),
+               "mass_rel_drift": max(abs(L.total_mass(f) - m0[f]) / m0[f] for f in range(2))}
+        log("%s: %s" % (storage, out))
+        return L, out
+
+    ref, res = run("f64")
+    fields = {"rho": [ref.density(f) for f in range(2)], "u": [ref.velocity(f) for f in range(2)]}
+    result = {"f64": res}
+    del ref
+    torch.cuda.empty_cache()
+    for st in storages:
+        L, res = run(st)
+        res["max_abs_drho"] = max(float(np.abs(L.density(f) - fields["rho"][f]).max())
+                                  for f in range(2))
+        res["max_abs_du"] = max(float(np.abs(L.velocity(f) - fields["u"][f]).max())
+                                for f in range(2))
+        umax = max(float(np.abs(fields["u"][f]).max()) for f in range(2))
+        res["max_abs_du_over_max_u"] = res["max_abs_du"] / umax
+        res["radius_rel_diff"] = (res["laplace"]["radius"] / result["f64"]["laplace"]["radius"]
+                                  - 1.0)
+        result[st] = res
+        del L
+        torch.cuda.empty_cache()
+    print(json.dumps({"workload": "D3Q19 two-component Shan-Chen droplet %d^3, radius %.0f, "
+                                  "1.0/0.05, BDist2, tau 1, g 3.6, %d steps" % (n, radius, steps),
+                      "results": result}), flush=True)
+
+
+def ensemble_params(count=256, seed=20261202):
+    """Independent droplet lattices: cubic L ~ U{32..96}, tau ~ U[1.0, 1.5]
+    (both components), g ~ U[3.0, 4.5], radius ~ U[0.2, 0.35] * L
+    (EqDistSphereFrac, 0.7/0.04, interface width 5), BDist2."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(rng.integers(32, 97))
+        tau = float(rng.uniform(1.0, 1.5))
+        g = float(rng.uniform(3.0, 4.5))
+        frac = float(rng.uniform(0.2, 0.35))
+        out.append(dict(d=3, q=19, gn=[n, n, n], components=2, tau=[tau, tau],
+                        eqDistForm="BDist2", gcc=[[0.0, g], [g, 0.0]], enableShanChen=True,
+                        psiForm="Exponential", fluidInit=["EqDistSphereFrac"] * 2,
+                        fluidDensities=[[0.7, 0.04], [0.04, 0.7]], initRadius=frac,
+                        interfaceThickness=5.0, rho0=[0.37, 0.37]))
+    return out
+
+
+def ensemble_workload(D, count, steps, storage="f32_shift"):
+    """The ensemble advanced by one persistent launch per step (graph replay,
+    autotuned tiles) and by one launch per lattice and step; MLUPS counts
+    sum(L^3) per step."""
+    import torch
+    params = ensemble_params(count)
+    t0 = time.perf_counter()
+    lats = [D.Lattice.from_params(p, storage=storage) for p in params]
+    sites = sum(int(np.prod(p["gn"])) for p in params)
+    log("ensemble: %d lattices, %d sites, inputs %.1f s"
+        % (count, sites, time.perf_counter() - t0))
+    B = D.Batch(lats)
+    cfg, ms = B.autotune()
+    log("autotuned (tile rows, chunk, workgroups) = %s, %.3f ms per launch" % (cfg, ms))
+    B.step(2)
+    B.synchronize()
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        fn()
+        B.synchronize()
+        return time.perf_counter() - t
+    dt_graph = timed(lambda: B.step(steps, graph=True))
+    dt_eager = timed(lambda: B.step(steps, graph=False))
+    dt_loop = timed(lambda: B.step_looped(steps))
+    finite = all(np.isfinite(L.total_mass(0)) for L in lats)
+    res = {"lattices": count, "sites": sites, "steps": steps, "storage": storage,
+           "config": {"tile_rows": cfg[0], "chunk": cfg[1], "workgroups": cfg[2],
+                      "kernel": ["solid-capable", "no-solid"][cfg[3]]},
+           "autotune_launch_ms": round(ms, 4),
+           "batched_graph_mlups": round(sites * steps / dt_graph / 1e6, 1),
+           "batched_eager_mlups": round(sites * steps / dt_eager / 1e6, 1),
+           "looped_mlups": round(sites * steps / dt_loop / 1e6, 1),
+           "speedup_vs_looped": round(dt_loop / dt_graph, 3), "finite": bool(finite)}
+    print(json.dumps({"workload": "ensemble of %d D3Q19 two-component droplet lattices, "
+                                  "L ~ U{32..96}" % count, "results": res}), flush=True)
+
+
 def copy_calibration(gib=2.0, reps=5):
     """A plain streaming kernel of known traffic (torch.neg over FP64 of
     `gib` GiB: every byte read once and written once), for scaling the
@@ -204,6 +331,11 @@ def main():
     ap.add_argument("--schedule", default="auto", choices=["auto", "two_pass", "fused"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-storage-legs", action="store_true")
+    ap.add_argument("--workload", default="channel", choices=["channel", "droplet", "ensemble"],
+                    help="channel: the MLUPS line; droplet: shifted-storage errors against "
+                         "FP64 on a centred droplet (radius 96 at 512^3, scaled with --size); "
+                         "ensemble: 256 uneven lattices, batched vs looped")
+    ap.add_argument("--lattices", type=int, default=256, help="ensemble size")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="time a plain FP64 streaming kernel of known size and exit")
     args = ap.parse_args()
@@ -217,6 +349,12 @@ def main():
         sys.exit("bench.py: no GPU visible")
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.workload == "droplet":
+        droplet_workload(D, args.size, args.steps)
+        return
+    if args.workload == "ensemble":
+        ensemble_workload(D, args.lattices, args.steps)
         return
     n = args.size
     p = channel_params(n)

@@ -13,5 +13,5 @@ There is no CPU fallback: every call goes through the HIP library, and a
 missing library raises.
 """
 from .lattice import (  # noqa: F401
-    DlbcError, Lattice, Rng, init_fluid, init_mask, init_eq_dist_wall, lib, lib_path,
+    Batch, DlbcError, Lattice, Rng, init_fluid, init_mask, init_eq_dist_wall, lib, lib_path,
     EQ_FORMS, PSI_FORMS, FLUID_INIT, MASK_INIT, AXIS, STORAGE, SCHEDULES)

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/dlbc_hip.h"
+#include "kernels_batch.hpp"
 #include "kernels_fused.hpp"
 #include "kernels_shifted.hpp"
 #include "kernels_twopass.hpp"
@@ -42,6 +43,13 @@ struct dlbc_gpu_ctx {
   StepParams p{};
   int schedule = DLBC_SCHED_AUTO;
   int target_blocks = 0;  // fused kernels: minimum workgroups per launch (0: default)
+  // shifted storage: the tiled kernel over this lattice alone (a batch of one)
+  void *bufX = nullptr, *bufY = nullptr;  // the two population buffers as allocated
+  BatchLattice *d_self = nullptr;
+  BatchLattice self_tab{};
+  bool self_valid = false;
+  BatchItem *d_items = nullptr;
+  int nitems = 0, items_target = -1;
   std::string err;
   int64_t dev_bytes = 0;
   hipEvent_t ev[2 * kMaxKernels + 2] = {};
@@ -341,6 +349,51 @@ struct Timer {
   }
 };
 
+// One step of a shifted-storage lattice: the tiled kernel of a batch over
+// this lattice's own work list (8 x 32 tiles, x cut into chunks until the
+// launch holds at least target_blocks workgroups, 512 by default: FP32
+// arithmetic leaves room for two 512-thread workgroups per CU).
+int launch_shifted_self(dlbc_gpu_ctx *c) {
+  const int target = c->target_blocks > 0 ? c->target_blocks : 512;
+  if (c->items_target != target) {
+    const int tiles = ((c->geo.ny + 7) / 8) * ((c->geo.nz + 31) / 32);
+    const int want = std::max(1, std::min((target + tiles - 1) / tiles, c->geo.nx / 4));
+    const int chunk = (c->geo.nx + want - 1) / want;
+    std::vector<BatchItem> items;
+    batch_items(&c->geo, 1, 8, chunk, items);
+    if (c->d_items) HIPCHK(c, hipFree(c->d_items));
+    c->d_items = nullptr;
+    HIPCHK(c, hipMalloc(&c->d_items, sizeof(BatchItem) * items.size()));
+    HIPCHK(c, hipMemcpyAsync(c->d_items, items.data(), sizeof(BatchItem) * items.size(),
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->nitems = (int)items.size();
+    c->items_target = target;
+  }
+  BatchLattice t;
+  std::memset(&t, 0, sizeof t);
+  t.pop[0] = c->bufX;
+  t.pop[1] = c->bufY;
+  t.nbw = c->nbw;
+  t.S = c->S;
+  t.g = c->geo;
+  t.p = c->sparams;
+  if (!c->self_valid || std::memcmp(&t, &c->self_tab, sizeof t)) {
+    HIPCHK(c, hipMemcpyAsync(c->d_self, &t, sizeof t, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->self_tab = t;
+    c->self_valid = true;
+  }
+  const int parity = c->popA == c->bufX ? 0 : 1;
+  const int grid = (c->nitems + 7) / 8 * 8;
+  const int rc = c->storage == DLBC_STORAGE_F32_SHIFT
+                     ? launch_batch_d3q19<float>(c->d_self, c->d_items, c->nitems, parity, 8,
+                                                 grid, true, c->stream)
+                     : launch_batch_d3q19<__half>(c->d_self, c->d_items, c->nitems, parity, 8,
+                                                  grid, true, c->stream);
+  return rc;
+}
+
 template <class L, int C>
 struct TwoPassStep {
   static int run(dlbc_gpu_ctx *c, Timer *tm, double *acc_ms) {
@@ -374,12 +427,8 @@ struct FusedStep {
         rc = launch_fused_d3q19(static_cast<const double *>(c->popA),
                                 static_cast<double *>(c->popB), c->nbw, c->S, c->geo, c->p,
                                 c->target_blocks, c->stream);
-      else if (c->storage == DLBC_STORAGE_F32_SHIFT)
-        rc = launch_shifted_d3q19<float>(c->popA, c->popB, c->nbw, c->S, c->geo, c->sparams,
-                                         c->target_blocks, c->stream);
       else
-        rc = launch_shifted_d3q19<__half>(c->popA, c->popB, c->nbw, c->S, c->geo, c->sparams,
-                                          c->target_blocks, c->stream);
+        rc = launch_shifted_self(c);
       tm->after();
       if (rc) return fail(c, DLBC_ERR_UNSUPPORTED, "fused kernel: lattice shape not supported");
       std::swap(c->popA, c->popB);
@@ -645,9 +694,8 @@ int dlbc_gpu_create(dlbc_gpu_ctx **out, int d, int q, int components, const int6
     dlbc_gpu_destroy(c);
     return code;
   };
-  // shifted storage runs on the fused D3Q19 kernel only: n_y % 8 == 0, n_z % 32 == 0
-  if (storage != DLBC_STORAGE_F64 && !fused_d3q19_supported(c->geo))
-    return bail(DLBC_ERR_UNSUPPORTED);
+  if (storage != DLBC_STORAGE_F64 && (c->n[0] < 4 || c->n[1] < 4 || c->n[2] < 4))
+    return bail(DLBC_ERR_UNSUPPORTED);  // the tiled kernel needs n >= 4 on every axis
   c->elem = storage == DLBC_STORAGE_F64 ? 8 : (storage == DLBC_STORAGE_F32_SHIFT ? 4 : 2);
   if (hipSetDevice(device) != hipSuccess) return bail(DLBC_ERR_HIP);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
@@ -655,6 +703,10 @@ int dlbc_gpu_create(dlbc_gpu_ctx **out, int d, int q, int components, const int6
   const size_t popb = (size_t)c->elem * components * q * c->geo.sp;
   if (hipMalloc(&c->popA, popb) != hipSuccess) return bail(DLBC_ERR_NOMEM);
   if (hipMalloc(&c->popB, popb) != hipSuccess) return bail(DLBC_ERR_NOMEM);
+  c->bufX = c->popA;
+  c->bufY = c->popB;
+  if (storage != DLBC_STORAGE_F64 && hipMalloc(&c->d_self, sizeof(BatchLattice)) != hipSuccess)
+    return bail(DLBC_ERR_NOMEM);
   if (hipMalloc(&c->S, sizeof(double) * components * c->geo.sp) != hipSuccess)
     return bail(DLBC_ERR_NOMEM);
   if (hipMalloc(&c->mask, c->geo.sp) != hipSuccess) return bail(DLBC_ERR_NOMEM);
@@ -894,7 +946,8 @@ int dlbc_gpu_synchronize(dlbc_gpu_ctx *c) {
 
 int dlbc_gpu_set_schedule(dlbc_gpu_ctx *c, int schedule) {
   if (!c || schedule < 0 || schedule > 2) return DLBC_ERR_ARG;
-  if (schedule == DLBC_SCHED_FUSED && !(c->d == 3 && c->C == 2 && fused_d3q19_supported(c->geo)))
+  if (schedule == DLBC_SCHED_FUSED && c->storage == DLBC_STORAGE_F64 &&
+      !(c->d == 3 && c->C == 2 && fused_d3q19_supported(c->geo)))
     return fail(c, DLBC_ERR_UNSUPPORTED,
                 "fused schedule needs D3Q19, 2 components, n_y %% 8 == 0 and n_z %% 32 == 0");
   if (schedule == DLBC_SCHED_TWO_PASS && c->storage != DLBC_STORAGE_F64)
@@ -1036,10 +1089,328 @@ void dlbc_gpu_destroy(dlbc_gpu_ctx *c) {
   if (c->mask) (void)hipFree(c->mask);
   if (c->nbw) (void)hipFree(c->nbw);
   if (c->scratch) (void)hipFree(c->scratch);
+  if (c->d_self) (void)hipFree(c->d_self);
+  if (c->d_items) (void)hipFree(c->d_items);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
+}
+
+}  // extern "C"
+
+// ============================================================================
+// Batches of lattices: one persistent launch per step over all their tiles
+// ============================================================================
+struct dlbc_batch {
+  std::vector<dlbc_gpu_ctx *> ctx;
+  int storage = 0, device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr, e0 = nullptr, e1 = nullptr;
+  BatchLattice *d_lats = nullptr;
+  std::vector<BatchLattice> h_lats;  // the table on the device
+  bool lats_valid = false;
+  BatchItem *d_items = nullptr;
+  size_t items_cap = 0;
+  std::vector<BatchItem> h_items;
+  std::vector<int> lat_first;  // first item of each lattice, and the total
+  int ty = 8, chunk = 1 << 20, grid = 512;
+  int kernel = 0;  // 0: the solid-capable kernel, 1: the kernel for lattices without solids
+  bool items_dirty = true;
+  hipGraphExec_t graph = nullptr;
+  std::string err;
+};
+
+namespace {
+
+int bfail(dlbc_batch *b, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  b->err = buf;
+  return code;
+}
+
+#define BHIPCHK(b, call)                                                                    \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return bfail((b), DLBC_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));       \
+  } while (0)
+
+void batch_drop_graph(dlbc_batch *b) {
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  b->graph = nullptr;
+}
+
+// Work list for the current tile height and chunk length, on the device.
+int batch_build_items(dlbc_batch *b) {
+  std::vector<Geo> geo;
+  for (auto *c : b->ctx) geo.push_back(c->geo);
+  b->h_items.clear();
+  b->lat_first.assign(1, 0);
+  for (size_t l = 0; l < geo.size(); ++l) {
+    std::vector<BatchItem> one;
+    batch_items(&geo[l], 1, b->ty, b->chunk, one);
+    for (auto it : one) {
+      it.lat = (int)l;
+      b->h_items.push_back(it);
+    }
+    b->lat_first.push_back((int)b->h_items.size());
+  }
+  if (b->h_items.size() > b->items_cap) {
+    if (b->d_items) BHIPCHK(b, hipFree(b->d_items));
+    b->d_items = nullptr;
+    BHIPCHK(b, hipMalloc(&b->d_items, sizeof(BatchItem) * b->h_items.size()));
+    b->items_cap = b->h_items.size();
+  }
+  BHIPCHK(b, hipMemcpyAsync(b->d_items, b->h_items.data(), sizeof(BatchItem) * b->h_items.size(),
+                            hipMemcpyHostToDevice, b->stream));
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  b->items_dirty = false;
+  batch_drop_graph(b);
+  return DLBC_OK;
+}
+
+// Make every member ready (solid densities, pending uploads done) and the
+// device table current (buffers swap between calls, parameters may change).
+int batch_prepare(dlbc_batch *b) {
+  std::vector<BatchLattice> t(b->ctx.size());
+  for (size_t l = 0; l < b->ctx.size(); ++l) {
+    dlbc_gpu_ctx *c = b->ctx[l];
+    int rc = prepare(c);
+    if (rc) return bfail(b, rc, "lattice %zu: %s", l, c->err.c_str());
+    BHIPCHK(b, hipStreamSynchronize(c->stream));
+    std::memset(&t[l], 0, sizeof(BatchLattice));
+    t[l].pop[0] = c->popA;
+    t[l].pop[1] = c->popB;
+    t[l].nbw = c->nbw;
+    t[l].S = c->S;
+    t[l].g = c->geo;
+    t[l].p = c->sparams;
+  }
+  if (!b->lats_valid || std::memcmp(t.data(), b->h_lats.data(), sizeof(BatchLattice) * t.size())) {
+    BHIPCHK(b, hipMemcpyAsync(b->d_lats, t.data(), sizeof(BatchLattice) * t.size(),
+                              hipMemcpyHostToDevice, b->stream));
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    b->h_lats = t;
+    b->lats_valid = true;
+  }
+  if (b->items_dirty) return batch_build_items(b);
+  return DLBC_OK;
+}
+
+bool batch_has_solid(const dlbc_batch *b) {
+  for (auto *c : b->ctx)
+    if (c->has_solid) return true;
+  return false;
+}
+
+int batch_launch(dlbc_batch *b, const BatchItem *items, int nitems, int parity, int grid) {
+  const bool solid_kernel = b->kernel == 0 || batch_has_solid(b);
+  const int rc = b->storage == DLBC_STORAGE_F32_SHIFT
+                     ? launch_batch_d3q19<float>(b->d_lats, items, nitems, parity, b->ty, grid,
+                                                 solid_kernel, b->stream)
+                     : launch_batch_d3q19<__half>(b->d_lats, items, nitems, parity, b->ty, grid,
+                                                  solid_kernel, b->stream);
+  if (rc) return bfail(b, DLBC_ERR_ARG, "unsupported tile height %d", b->ty);
+  BHIPCHK(b, hipGetLastError());
+  return DLBC_OK;
+}
+
+// After nsteps: members whose buffers advanced an odd number of steps swap
+// them, and later work on a member's own stream waits for the batch.
+int batch_finish(dlbc_batch *b, int nsteps) {
+  BHIPCHK(b, hipEventRecord(b->done, b->stream));
+  for (auto *c : b->ctx) {
+    if (nsteps & 1) std::swap(c->popA, c->popB);
+    BHIPCHK(b, hipStreamWaitEvent(c->stream, b->done, 0));
+  }
+  return DLBC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dlbc_batch_create(dlbc_batch **out, dlbc_gpu_ctx *const *lattices, int n) {
+  if (!out || !lattices || n < 1) return DLBC_ERR_ARG;
+  *out = nullptr;
+  const int storage = lattices[0] ? lattices[0]->storage : -1;
+  for (int l = 0; l < n; ++l) {
+    const dlbc_gpu_ctx *c = lattices[l];
+    if (!c) return DLBC_ERR_ARG;
+    if (c->d != 3 || c->q != 19 || c->C != 2 || c->storage == DLBC_STORAGE_F64 ||
+        c->storage != storage || c->device != lattices[0]->device)
+      return DLBC_ERR_UNSUPPORTED;
+    for (int k = 0; k < l; ++k)
+      if (lattices[k] == c) return DLBC_ERR_ARG;  // a lattice may appear once
+  }
+  auto *b = new dlbc_batch();
+  b->ctx.assign(lattices, lattices + n);
+  b->storage = storage;
+  b->device = lattices[0]->device;
+  auto bail = [&](int code) {
+    dlbc_batch_destroy(b);
+    return code;
+  };
+  if (hipSetDevice(b->device) != hipSuccess) return bail(DLBC_ERR_HIP);
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(DLBC_ERR_HIP);
+  if (hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreate(&b->e0) != hipSuccess || hipEventCreate(&b->e1) != hipSuccess)
+    return bail(DLBC_ERR_HIP);
+  if (hipMalloc(&b->d_lats, sizeof(BatchLattice) * n) != hipSuccess) return bail(DLBC_ERR_NOMEM);
+  *out = b;
+  return DLBC_OK;
+}
+
+int dlbc_batch_set_config(dlbc_batch *b, int tile_y, int chunk, int workgroups, int kernel) {
+  if (!b || (tile_y != 4 && tile_y != 8) || chunk < 4 || workgroups < 8 || kernel < 0 ||
+      kernel > 1)
+    return DLBC_ERR_ARG;
+  if (tile_y != b->ty || chunk != b->chunk) b->items_dirty = true;
+  if (workgroups != b->grid || kernel != b->kernel) batch_drop_graph(b);
+  b->ty = tile_y;
+  b->chunk = chunk;
+  b->grid = workgroups;
+  b->kernel = kernel;
+  return DLBC_OK;
+}
+
+int dlbc_batch_get_config(const dlbc_batch *b, int *out) {
+  if (!b || !out) return DLBC_ERR_ARG;
+  out[0] = b->ty;
+  out[1] = b->chunk;
+  out[2] = b->grid;
+  out[3] = b->kernel;
+  return DLBC_OK;
+}
+
+int dlbc_batch_step(dlbc_batch *b, int nsteps, int use_graph) {
+  if (!b || nsteps < 0) return DLBC_ERR_ARG;
+  if (nsteps == 0) return DLBC_OK;
+  int rc = batch_prepare(b);
+  if (rc) return rc;
+  const int nit = (int)b->h_items.size();
+  int t = 0;
+  if (use_graph && nsteps >= 2) {
+    if (!b->graph) {  // two steps: even parity, then odd
+      hipGraph_t g = nullptr;
+      BHIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      int rc0 = batch_launch(b, b->d_items, nit, 0, b->grid);
+      int rc1 = rc0 ? rc0 : batch_launch(b, b->d_items, nit, 1, b->grid);
+      const hipError_t ec = hipStreamEndCapture(b->stream, &g);
+      if (rc1) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc1;
+      }
+      BHIPCHK(b, ec);
+      const hipError_t ei = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      BHIPCHK(b, ei);
+    }
+    for (; t + 2 <= nsteps; t += 2) BHIPCHK(b, hipGraphLaunch(b->graph, b->stream));
+  }
+  for (; t < nsteps; ++t) {
+    rc = batch_launch(b, b->d_items, nit, t & 1, b->grid);
+    if (rc) return rc;
+  }
+  return batch_finish(b, nsteps);
+}
+
+int dlbc_batch_step_looped(dlbc_batch *b, int nsteps) {
+  if (!b || nsteps < 0) return DLBC_ERR_ARG;
+  if (nsteps == 0) return DLBC_OK;
+  int rc = batch_prepare(b);
+  if (rc) return rc;
+  for (int t = 0; t < nsteps; ++t)
+    for (size_t l = 0; l + 1 < b->lat_first.size(); ++l) {
+      const int first = b->lat_first[l], cnt = b->lat_first[l + 1] - first;
+      rc = batch_launch(b, b->d_items + first, cnt, t & 1, std::min(b->grid, (cnt + 7) / 8 * 8));
+      if (rc) return rc;
+    }
+  return batch_finish(b, nsteps);
+}
+
+int dlbc_batch_autotune(dlbc_batch *b, int *out, double *best_ms) {
+  if (!b) return DLBC_ERR_ARG;
+  int rc = batch_prepare(b);
+  if (rc) return rc;
+  const int tys[2] = {8, 4};
+  const int chunks[4] = {16, 32, 64, 1 << 20};
+  const int grids[4] = {256, 512, 1024, 2048};
+  double best = 1e300;
+  int bt = b->ty, bc = b->chunk, bg = b->grid, bk = b->kernel;
+  const int nkernels = batch_has_solid(b) ? 1 : 2;
+  for (int ty : tys)
+    for (int ch : chunks) {
+      b->ty = ty;
+      b->chunk = ch;
+      rc = batch_build_items(b);
+      if (rc) return rc;
+      const int nit = (int)b->h_items.size();
+      for (int kn = 0; kn < nkernels; ++kn)
+      for (int gr : grids) {
+        b->kernel = kn;
+        // parity 0 only: reads the current state, writes the other buffer
+        rc = batch_launch(b, b->d_items, nit, 0, gr);
+        if (rc) return rc;
+        BHIPCHK(b, hipEventRecord(b->e0, b->stream));
+        for (int k = 0; k < 3; ++k) {
+          rc = batch_launch(b, b->d_items, nit, 0, gr);
+          if (rc) return rc;
+        }
+        BHIPCHK(b, hipEventRecord(b->e1, b->stream));
+        BHIPCHK(b, hipEventSynchronize(b->e1));
+        float ms = 0.0f;
+        BHIPCHK(b, hipEventElapsedTime(&ms, b->e0, b->e1));
+        if (ms / 3.0 < best) {
+          best = ms / 3.0;
+          bt = ty;
+          bc = ch;
+          bg = gr;
+          bk = kn;
+        }
+      }
+    }
+  b->ty = bt;
+  b->chunk = bc;
+  b->grid = bg;
+  b->kernel = bk;
+  rc = batch_build_items(b);
+  if (rc) return rc;
+  if (out) {
+    out[0] = bt;
+    out[1] = bc;
+    out[2] = bg;
+    out[3] = bk;
+  }
+  if (best_ms) *best_ms = best;
+  return DLBC_OK;
+}
+
+int dlbc_batch_synchronize(dlbc_batch *b) {
+  if (!b) return DLBC_ERR_ARG;
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  return DLBC_OK;
+}
+
+const char *dlbc_batch_last_error(const dlbc_batch *b) { return b ? b->err.c_str() : ""; }
+
+void dlbc_batch_destroy(dlbc_batch *b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_drop_graph(b);
+  if (b->d_lats) (void)hipFree(b->d_lats);
+  if (b->d_items) (void)hipFree(b->d_items);
+  if (b->done) (void)hipEventDestroy(b->done);
+  if (b->e0) (void)hipEventDestroy(b->e0);
+  if (b->e1) (void)hipEventDestroy(b->e1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
 }
 
 }  // extern "C"

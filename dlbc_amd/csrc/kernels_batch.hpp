@@ -1,0 +1,391 @@
+// kernels_batch.hpp -- persistent D3Q19 two-component Shan-Chen step over a
+// list of tiles of one or many lattices, on shifted FP32 / FP16 storage.
+//
+// The per-tile schedule is the one of the fused kernels (kernels_fused.hpp):
+// a workgroup owns a TY x 32 tile of the (y, z) plane, marches along x over a
+// chunk of planes, keeps psi of three planes in an LDS ring and stages the
+// one-site ring around the tile two planes ahead.  Two things are added:
+//
+//   * partial tiles: a tile at the end of y or z covers vy <= TY rows and
+//     vz <= 32 columns; the psi ring is the border of the (vy+2) x (vz+2)
+//     box, so lattices of any size n >= 4 are handled;
+//   * a work list: the launch is persistent (a few workgroups per CU), each
+//     workgroup walks a list of items (lattice, tile, chunk), and every item
+//     carries its lattice's geometry, buffers and parameters.  A whole
+//     ensemble of independent lattices of uneven size and parameters thus
+//     advances in one launch per step.
+//
+// The arithmetic per site is the one of kernels_shifted.hpp (deviation form,
+// FP32), evaluated in the same order, so a lattice gives the same bits
+// whether it is stepped alone or inside a batch, and whatever the tile
+// height, chunk length or grid size.
+#pragma once
+
+#include <algorithm>
+#include <vector>
+
+#include "kernels_shifted.hpp"
+
+namespace dlbc {
+
+// One lattice of a batch.  A[0] is read and A[1] written on even steps of
+// the batch, the other way round on odd steps.
+struct BatchLattice {
+  const void *pop[2];
+  const uint32_t *nbw;  // solid-neighbour words (used when p.has_solid)
+  const double *S;      // density at solid sites (wall force)
+  Geo g;
+  ShiftParams p;
+};
+
+// A work item: lattice index, tile origin (y0, z0), planes [xs, xe).
+struct BatchItem {
+  int lat, y0, z0, xs, xe, pad0, pad1, pad2;
+};
+
+// Ring index r -> (ey, ez) on the border of the (vy+2) x (vz+2) box: the two
+// rows first, then the two columns.
+__device__ inline void ring_coord_v(int r, int vy, int vz, int &ey, int &ez) {
+  const int ez2 = vz + 2;
+  if (r < 2 * ez2) {
+    ey = r < ez2 ? 0 : vy + 1;
+    ez = r < ez2 ? r : r - ez2;
+  } else {
+    const int k = r - 2 * ez2;
+    ey = 1 + (k >> 1);
+    ez = (k & 1) ? vz + 1 : 0;
+  }
+}
+
+// One work item (a tile and a chunk of planes).  FULL: the tile is TY x 32
+// (ring size and coordinates known at compile time); SOLID: the lattice has
+// solid sites.
+template <class Store, int TY, bool FULL, bool SOLID>
+__device__ __forceinline__ void batch_tile(const BatchItem &item, const BatchLattice &B,
+                                           int parity, float (*psi_lds)[2][FusedShape<TY>::EY *
+                                                                           FusedShape<TY>::EZ],
+                                           float (*stage)[FusedShape<TY>::ring_items]) {
+  using L = D3Q19;
+  using Sh = FusedShape<TY>;
+  constexpr int Q = 19, TZ = Sh::TZ, EZ = Sh::EZ;
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int ly = t >> 6;
+  const int lz = lane & 31;
+  const int f = lane >> 5;
+  const int me_e = (ly + 1) * EZ + (lz + 1);
+  auto pslot = [](int x) { return x & 3; };
+  {
+    const Geo g = B.g;
+    // Parameters: uniform ones read from the table, per-component ones selected
+    // into registers once per item (a lane holds component f).
+    const ShiftParams &p = B.p;
+    auto sel = [f](float a0, float a1) { return f ? a1 : a0; };
+    const float tau_f = sel(p.tau[0], p.tau[1]), tau_o = sel(p.tau[1], p.tau[0]);
+    const float om_f = sel(p.omega[0], p.omega[1]);
+    const float g_self = sel(p.gcc[0][0], p.gcc[1][1]), g_other = sel(p.gcc[0][1], p.gcc[1][0]);
+    const float wc_f = sel(p.gwc[0], p.gwc[1]);
+    const float body_f[3] = {sel(p.body[0][0], p.body[1][0]), sel(p.body[0][1], p.body[1][1]),
+                             sel(p.body[0][2], p.body[1][2])};
+    const float rho0_0 = p.rho0[0], rho0_1 = p.rho0[1];
+    const int psi_form = p.psi_form;
+    const Store *__restrict__ A = static_cast<const Store *>(B.pop[parity]);
+    Store *__restrict__ W = static_cast<Store *>(const_cast<void *>(B.pop[parity ^ 1]));
+    const uint32_t *__restrict__ nbw = B.nbw;
+    constexpr bool solid = SOLID;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int64_t sp = g.sp;
+    const int plane = ny * nz;
+    const int y0 = item.y0, z0 = item.z0, xs = item.xs, xe = item.xe;
+    const int vy = FULL ? TY : min(TY, ny - y0), vz = FULL ? TZ : min(TZ, nz - z0);
+    const bool active = FULL || (ly < vy && lz < vz);
+    const int ring = FULL ? Sh::ring : 2 * (vz + 2) + 2 * vy;
+    const int ring_items = ring * 2 * Q;
+    const int y = y0 + ly, z = z0 + lz;
+    const Store *__restrict__ Af = A + (int64_t)f * Q * sp;
+    Store *__restrict__ Wf = W + (int64_t)f * Q * sp;
+    const float rho0 = sel(rho0_0, rho0_1);
+
+    auto gsite = [&](int x, int yy, int zz) { return (x * ny + yy) * nz + zz; };
+    // bit 31 set: nothing to do at this lane and plane (solid site or no site)
+    auto solid_word = [&](int x) -> uint32_t {
+      if (!active) return 1u << 31;
+      return solid ? nbw[gsite(wrapf(x, nx), y, z)] : 0u;
+    };
+    const int ym = active ? wrapf(y - 1, ny) : 0, yp = active ? wrapf(y + 1, ny) : 0;
+    const int zm = active ? wrapf(z - 1, nz) : 0, zpl = active ? wrapf(z + 1, nz) : 0;
+    auto pull_own = [&](int x, uint32_t sw, float *n) {
+      const int xm = wrapf(x - 1, nx), x0 = wrapf(x, nx), xp = wrapf(x + 1, nx);
+      const int self = gsite(x0, y, z);
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        const int cx = L::c[i][0], cy = L::c[i][1], cz = L::c[i][2];
+        const int sx = cx == 1 ? xm : (cx == -1 ? xp : x0);
+        const int sy = cy == 1 ? ym : (cy == -1 ? yp : y);
+        const int sz = cz == 1 ? zm : (cz == -1 ? zpl : z);
+        const bool bb = (sw >> L::bounce[i]) & 1u;
+        const int comp = bb ? L::bounce[i] : i;
+        const int src = bb ? self : gsite(sx, sy, sz);
+        n[i] = to_f(Af[(int64_t)comp * sp + src]);
+      }
+    };
+    auto drho = [](const float *n) {
+      float s = 0.0f;
+#pragma unroll
+      for (int i = 0; i < Q; ++i) s += n[i];
+      return s;
+    };
+    auto stage_ring = [&](int x, float *stg) {
+      for (int w = t; w < ring_items; w += Sh::threads) {
+        const int fq = w / ring, r = w - fq * ring;
+        const int fr = fq >= Q ? 1 : 0, i = fq - fr * Q;
+        int ey, ez;
+        if constexpr (FULL)
+          ring_coord<Sh>(r, ey, ez);
+        else
+          ring_coord_v(r, vy, vz, ey, ez);
+        const int yy = wrapf(y0 - 1 + ey, ny), zz = wrapf(z0 - 1 + ez, nz);
+        const int cx = d3q19_c(i, 0), cy = d3q19_c(i, 1), cz = d3q19_c(i, 2);
+        const int self = gsite(wrapf(x, nx), yy, zz);
+        const int bi = d3q19_bounce(i);
+        const uint32_t rw = solid ? nbw[self] : 0u;
+        float v = 0.0f;
+        if (!(rw >> 31)) {
+          const bool bb = (rw >> bi) & 1u;
+          const int comp = bb ? bi : i;
+          const int src =
+              bb ? self : gsite(wrapf(x - cx, nx), wrapf(yy - cy, ny), wrapf(zz - cz, nz));
+          v = to_f(A[(int64_t)(fr * Q + comp) * sp + src]);
+        }
+        stg[w] = v;
+      }
+    };
+    auto reduce_ring = [&](int x, const float *stg) {
+      if (t < 2 * ring) {
+        const int fr = t >= ring ? 1 : 0, r = t - fr * ring;
+        int ey, ez;
+        if constexpr (FULL)
+          ring_coord<Sh>(r, ey, ez);
+        else
+          ring_coord_v(r, vy, vz, ey, ez);
+        float s = 0.0f;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) s += stg[(fr * Q + i) * ring + r];
+        psi_lds[pslot(x)][fr][ey * EZ + ez] = psi_f(psi_form, (fr ? rho0_1 : rho0_0) + s);
+      }
+    };
+
+    float cur[Q];
+    float rho_cur = 0.0f;
+    uint32_t sw_cur;
+    {
+      const uint32_t sw = solid_word(xs - 1);
+      float n[Q];
+      if (!(sw >> 31)) {
+        pull_own(xs - 1, sw, n);
+        psi_lds[pslot(xs - 1)][f][me_e] = psi_f(psi_form, rho0 + drho(n));
+      }
+      stage_ring(xs - 1, stage[0]);
+      __syncthreads();
+      reduce_ring(xs - 1, stage[0]);
+      sw_cur = solid_word(xs);
+      if (!(sw_cur >> 31)) {
+        pull_own(xs, sw_cur, cur);
+        rho_cur = rho0 + drho(cur);
+        psi_lds[pslot(xs)][f][me_e] = psi_f(psi_form, rho_cur);
+      }
+      stage_ring(xs, stage[1]);
+      __syncthreads();
+      reduce_ring(xs, stage[1]);
+      __syncthreads();
+      stage_ring(xs + 1, stage[(xs + 1) & 1]);
+      __syncthreads();
+    }
+
+    uint32_t sw_ahead = solid_word(xs + 1);
+    for (int x = xs; x < xe; ++x) {
+      float nxt[Q];
+      float rho_nxt = 0.0f;
+      const uint32_t sw_nxt = sw_ahead;
+      sw_ahead = solid_word(x + 2);
+      if (!(sw_nxt >> 31)) {
+        pull_own(x + 1, sw_nxt, nxt);
+        rho_nxt = rho0 + drho(nxt);
+        psi_lds[pslot(x + 1)][f][me_e] = psi_f(psi_form, rho_nxt);
+      }
+      reduce_ring(x + 1, stage[(x + 1) & 1]);
+      stage_ring(x + 2, stage[x & 1]);
+      __syncthreads();
+      {
+        float j[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) j[d] = 0.0f;
+#pragma unroll
+        for (int i = 0; i < Q; ++i)
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+            if (L::c[i][d] != 0) j[d] += (float)L::c[i][d] * cur[i];
+        const float rho_p = shfl_xor32f(rho_cur);
+        float j_p[3];
+        const bool bdist2 = p.eq_form == kBDist2;
+        if (bdist2) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) j_p[d] = shfl_xor32f(j[d]);
+        }
+        if (!(sw_cur >> 31)) {
+          float F[3] = {0.0f, 0.0f, 0.0f};
+          const float psi1 = psi_lds[pslot(x)][f][me_e];
+          if (p.sc) {
+            // the other component first, then the own one: for the usual
+            // coupling matrix (zero diagonal) both lanes of a site take the
+            // same branch
+#pragma unroll
+            for (int which = 0; which < 2; ++which) {
+              const int f2 = which ? f : (f ^ 1);
+              const float cc = which ? g_self : g_other;
+              if (cc == 0.0f) continue;
+              const float psi_self = psi_lds[pslot(x)][f2][me_e];
+              float acc[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+              for (int i = 1; i < Q; ++i) {
+                const bool sn = (sw_cur >> i) & 1u;
+                const float psi2 =
+                    sn ? psi_self
+                       : psi_lds[pslot(x + L::c[i][0])][f2][me_e + L::c[i][1] * EZ + L::c[i][2]];
+                const float wp = (float)L::w[i] * psi2;
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                  if (L::c[i][d] != 0) acc[d] += (float)L::c[i][d] * wp;
+              }
+#pragma unroll
+              for (int d = 0; d < 3; ++d) F[d] -= psi1 * cc * acc[d];
+            }
+            const float wc = wc_f;
+            if (solid && wc != 0.0f) {
+#pragma unroll
+              for (int i = 0; i < Q; ++i) {
+                if ((sw_cur >> i) & 1u) {
+                  const int nb = gsite(wrapf(x + L::c[i][0], nx), wrapf(y + L::c[i][1], ny),
+                                       wrapf(z + L::c[i][2], nz));
+                  const float pf = (float)L::w[i] * psi1 * (float)B.S[f * sp + nb] * wc;
+#pragma unroll
+                  for (int d = 0; d < 3; ++d) F[d] -= pf * (float)L::c[i][d];
+                }
+              }
+            }
+          }
+          if (p.has_body) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) F[d] += body_f[d];
+          }
+          const float rinv = 1.0f / rho_cur;
+          float u[3];
+          if (bdist2) {
+            const float tot_inv = 1.0f / (rho_cur + rho_p);
+            const float tau_p = tau_o;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+              const float wv = (j[d] / tau_f + j_p[d] / tau_p) * tot_inv;
+              u[d] = tau_f * (p.gacc[d] + F[d] * rinv) + wv;
+            }
+          } else {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) u[d] = tau_f * (p.gacc[d] + F[d] * rinv) + j[d] * rinv;
+          }
+          float e[Q];
+          eq_dev_d3q19(p.eq_form, u, e);
+          const float drh = rho_cur - rho0;
+          const float om = om_f;
+          const int s = x * plane + y * nz + z;
+#pragma unroll
+          for (int i = 0; i < Q; ++i) {
+            const float deq = (float)L::w[i] * drh + rho_cur * e[i];
+            store_nt<Store>(&Wf[(int64_t)i * sp + s], cur[i] + om * (deq - cur[i]));
+          }
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < Q; ++i) cur[i] = nxt[i];
+      rho_cur = rho_nxt;
+      sw_cur = sw_nxt;
+    }
+    __syncthreads();  // the next item reuses psi_lds and stage
+  }
+}
+
+// SOLID: the solid-capable kernel (any lattice: one without solid sites has
+// all-zero solid words and takes the same arithmetic path).  It fits in 128
+// VGPRs, i.e. two 512-thread workgroups per CU.  !SOLID: lattices without
+// solid sites only; the compiler hoists more loads and needs ~166 VGPRs (one
+// 512-thread workgroup per CU).  The autotuner picks between them.
+template <class Store, int TY, bool SOLID>
+__global__ void __launch_bounds__(64 * TY, SOLID ? 4 : 1)
+    k_batch_d3q19(const BatchLattice *__restrict__ lats, const BatchItem *__restrict__ items,
+                  int nitems, int parity, int per_xcd) {
+  using Sh = FusedShape<TY>;
+  __shared__ float psi_lds[4][2][Sh::EY * Sh::EZ];
+  __shared__ float stage[2][Sh::ring_items];
+  // XCD-aware walk: workgroup b runs on XCD b % 8; the item list is cut into
+  // eight contiguous segments (neighbouring tiles), one per XCD, and the
+  // per_xcd workgroups of an XCD stride through its segment.
+  const int xcd = blockIdx.x & 7, j = blockIdx.x >> 3;
+  const int seg = (nitems + 7) >> 3;
+  const int it0 = xcd * seg + j, it1 = min(nitems, (xcd + 1) * seg);
+  for (int it = it0; it < it1; it += per_xcd) {
+    const BatchItem item = items[it];
+    const BatchLattice &B = lats[item.lat];
+    const bool full = item.y0 + TY <= B.g.ny && item.z0 + Sh::TZ <= B.g.nz;
+    if (full)
+      batch_tile<Store, TY, true, SOLID>(item, B, parity, psi_lds, stage);
+    else
+      batch_tile<Store, TY, false, SOLID>(item, B, parity, psi_lds, stage);
+  }
+}
+
+// Work list of a set of lattices: tiles of TY x 32 (partial at the ends of y
+// and z) and x chunks of at most `chunk` planes (at least 4), lattice-major,
+// then chunk, then tile row, then tile column.
+inline void batch_items(const Geo *g, int nlat, int ty, int chunk, std::vector<BatchItem> &out) {
+  out.clear();
+  for (int l = 0; l < nlat; ++l) {
+    const int nx = g[l].nx, ny = g[l].ny, nz = g[l].nz;
+    const int c = std::max(4, std::min(chunk, nx));
+    const int nch = (nx + c - 1) / c;
+    const int len = (nx + nch - 1) / nch;  // balanced chunks
+    for (int k = 0; k < nch; ++k) {
+      const int xs = k * len, xe = std::min(nx, xs + len);
+      if (xs >= xe) continue;
+      for (int y0 = 0; y0 < ny; y0 += ty)
+        for (int z0 = 0; z0 < nz; z0 += 32) out.push_back({l, y0, z0, xs, xe, 0, 0, 0});
+    }
+  }
+}
+
+template <class Store>
+inline int launch_batch_d3q19(const BatchLattice *lats, const BatchItem *items, int nitems,
+                              int parity, int ty, int grid, bool solid_kernel,
+                              hipStream_t stream) {
+  if (nitems <= 0) return 0;
+  grid = std::max(8, grid / 8 * 8);
+  int per_xcd = grid / 8;
+  const int seg = (nitems + 7) / 8;
+  if (per_xcd > seg) per_xcd = seg;  // no idle workgroups
+  grid = per_xcd * 8;
+  if (ty == 8 && solid_kernel)
+    hipLaunchKernelGGL((k_batch_d3q19<Store, 8, true>), dim3(grid), dim3(512), 0, stream, lats,
+                       items, nitems, parity, per_xcd);
+  else if (ty == 8)
+    hipLaunchKernelGGL((k_batch_d3q19<Store, 8, false>), dim3(grid), dim3(512), 0, stream, lats,
+                       items, nitems, parity, per_xcd);
+  else if (ty == 4 && solid_kernel)
+    hipLaunchKernelGGL((k_batch_d3q19<Store, 4, true>), dim3(grid), dim3(256), 0, stream, lats,
+                       items, nitems, parity, per_xcd);
+  else if (ty == 4)
+    hipLaunchKernelGGL((k_batch_d3q19<Store, 4, false>), dim3(grid), dim3(256), 0, stream, lats,
+                       items, nitems, parity, per_xcd);
+  else
+    return -1;
+  return 0;
+}
+
+}  // namespace dlbc

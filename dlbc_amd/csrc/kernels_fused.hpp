@@ -281,7 +281,11 @@ __global__ void __launch_bounds__(64 * TY)
               [&](int ff, int d) { return ff == f ? mom[d] : mom_p[d]; }, total, p, wv);
         const int s = xw * plane + y * nz + z;
         collide_component<L>(f, cur, rho_cur, total, F, wv, p,
-                             [&](int i, double v) { Bf[(int64_t)i * sp + s] = v; });
+                             [&](int i, double v) {
+                               // streaming store: the new populations are not read again
+                               // in this step, so they bypass the caches
+                               __builtin_nontemporal_store(v, &Bf[(int64_t)i * sp + s]);
+                             });
       }
     }
 #pragma unroll

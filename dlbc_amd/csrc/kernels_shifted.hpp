@@ -11,10 +11,9 @@
 //   j    = sum_i d_i c_i                    (sum_i w_i c_i = 0)
 //   d_i^eq = w_i (rho - rho0) + rho (E_i(u) - w_i)
 //   d_i'  = d_i + omega (d_i^eq - d_i)
-// where E_i(u) is the equilibrium of unit density.  The schedule is the one
-// of the FP64 fused kernel (kernels_fused.hpp): x-marching 8 x 32 tiles, two
-// lanes per site (one per component), psi in an LDS ring of four planes, the
-// one-site psi ring staged two planes ahead, one barrier per plane.
+// where E_i(u) is the equilibrium of unit density.  This file holds the
+// storage conversions, parameters and per-site pieces; the kernel that steps
+// shifted lattices (alone or in batches) is in kernels_batch.hpp.
 #pragma once
 
 #include <hip/hip_fp16.h>
@@ -42,6 +41,19 @@ __device__ inline float from_f<float>(float v) {
 template <>
 __device__ inline __half from_f<__half>(float v) {
   return __float2half_rn(v);
+}
+
+// Streaming (nontemporal) store of one population in the storage type.
+template <class T>
+__device__ inline void store_nt(T *p, float v);
+template <>
+__device__ inline void store_nt<float>(float *p, float v) {
+  __builtin_nontemporal_store(v, p);
+}
+template <>
+__device__ inline void store_nt<__half>(__half *p, float v) {
+  __builtin_nontemporal_store(__half_as_ushort(__float2half_rn(v)),
+                              reinterpret_cast<unsigned short *>(p));
 }
 
 struct ShiftParams {
@@ -108,260 +120,5 @@ __device__ inline void eq_dev_d3q19(int form, const float (&u)[3], float (&e)[19
 }
 
 __device__ inline float shfl_xor32f(float v) { return __shfl_xor(v, 32); }
-
-template <class Store, int TY, bool SOLID>
-__global__ void __launch_bounds__(64 * TY)
-    k_shifted_d3q19(const Store *__restrict__ A, Store *__restrict__ B,
-                    const uint32_t *__restrict__ nbw, const double *__restrict__ S, Geo g,
-                    ShiftParams p, int chunk, int ntiles_y, int ntiles_z, int nchunks) {
-  using L = D3Q19;
-  using Sh = FusedShape<TY>;
-  constexpr int Q = 19, TZ = Sh::TZ;
-  __shared__ float psi_lds[4][2][Sh::EY * Sh::EZ];
-  __shared__ float stage[2][Sh::ring_items];
-
-  const int nblocks = gridDim.x;
-  const int b = blockIdx.x;
-  const int logical = (nblocks % 8 == 0) ? (b % 8) * (nblocks / 8) + b / 8 : b;
-  const int tz = logical % ntiles_z;
-  const int ty = (logical / ntiles_z) % ntiles_y;
-  const int chunk_id = logical / (ntiles_z * ntiles_y);
-  if (chunk_id >= nchunks) return;
-  const int xs = chunk_id * chunk;
-  const int xe = min(xs + chunk, g.nx);
-  const int y0 = ty * TY, z0 = tz * TZ;
-  const int t = threadIdx.x;
-  const int lane = t & 63;
-  const int ly = t >> 6;
-  const int lz = lane & 31;
-  const int f = lane >> 5;
-  const int y = y0 + ly, z = z0 + lz;
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const int64_t sp = g.sp;
-  const int plane = ny * nz;
-  const Store *__restrict__ Af = A + (int64_t)f * Q * sp;
-  Store *__restrict__ Bf = B + (int64_t)f * Q * sp;
-  const float rho0 = p.rho0[f];
-
-  auto gsite = [&](int x, int yy, int zz) { return (x * ny + yy) * nz + zz; };
-  auto pslot = [](int x) { return x & 3; };
-  auto solid_word = [&](int x) -> uint32_t {
-    if constexpr (SOLID) {
-      return nbw[gsite(wrapf(x, nx), y, z)];
-    } else {
-      return 0u;
-    }
-  };
-  const int ym = wrapf(y - 1, ny), yp = wrapf(y + 1, ny);
-  const int zm = wrapf(z - 1, nz), zpl = wrapf(z + 1, nz);
-  auto pull_own = [&](int x, uint32_t sw, float *n) {
-    const int xm = wrapf(x - 1, nx), x0 = wrapf(x, nx), xp = wrapf(x + 1, nx);
-    const int self = gsite(x0, y, z);
-#pragma unroll
-    for (int i = 0; i < Q; ++i) {
-      const int cx = L::c[i][0], cy = L::c[i][1], cz = L::c[i][2];
-      const int sx = cx == 1 ? xm : (cx == -1 ? xp : x0);
-      const int sy = cy == 1 ? ym : (cy == -1 ? yp : y);
-      const int sz = cz == 1 ? zm : (cz == -1 ? zpl : z);
-      const bool bb = SOLID && ((sw >> L::bounce[i]) & 1u);
-      const int comp = bb ? L::bounce[i] : i;
-      const int src = bb ? self : gsite(sx, sy, sz);
-      n[i] = to_f(Af[(int64_t)comp * sp + src]);
-    }
-  };
-  auto drho = [](const float *n) {
-    float s = 0.0f;
-#pragma unroll
-    for (int i = 0; i < Q; ++i) s += n[i];
-    return s;
-  };
-  auto stage_ring = [&](int x, float *stg) {
-    for (int w = t; w < Sh::ring_items; w += Sh::threads) {
-      const int fq = w / Sh::ring, r = w - fq * Sh::ring;
-      const int fr = fq >= Q ? 1 : 0, i = fq - fr * Q;
-      int ey, ez;
-      ring_coord<Sh>(r, ey, ez);
-      const int yy = wrapf(y0 - 1 + ey, ny), zz = wrapf(z0 - 1 + ez, nz);
-      const int cx = d3q19_c(i, 0), cy = d3q19_c(i, 1), cz = d3q19_c(i, 2);
-      const int self = gsite(wrapf(x, nx), yy, zz);
-      const int bi = d3q19_bounce(i);
-      const uint32_t rw = SOLID ? nbw[self] : 0u;
-      float v = 0.0f;
-      if (!(rw >> 31)) {
-        const bool bb = (rw >> bi) & 1u;
-        const int comp = bb ? bi : i;
-        const int src = bb ? self : gsite(wrapf(x - cx, nx), wrapf(yy - cy, ny), wrapf(zz - cz, nz));
-        v = to_f(A[(int64_t)(fr * Q + comp) * sp + src]);
-      }
-      stg[w] = v;
-    }
-  };
-  auto reduce_ring = [&](int x, const float *stg) {
-    if (t < 2 * Sh::ring) {
-      const int fr = t >= Sh::ring ? 1 : 0, r = t - fr * Sh::ring;
-      int ey, ez;
-      ring_coord<Sh>(r, ey, ez);
-      float s = 0.0f;
-#pragma unroll
-      for (int i = 0; i < Q; ++i) s += stg[(fr * Q + i) * Sh::ring + r];
-      psi_lds[pslot(x)][fr][ey * Sh::EZ + ez] = psi_f(p.psi_form, p.rho0[fr] + s);
-    }
-  };
-  const int me_e = (ly + 1) * Sh::EZ + (lz + 1);
-
-  float cur[Q];
-  float rho_cur = 0.0f;
-  uint32_t sw_cur;
-  {
-    const uint32_t sw = solid_word(xs - 1);
-    float n[Q];
-    if (!(sw >> 31)) {
-      pull_own(xs - 1, sw, n);
-      psi_lds[pslot(xs - 1)][f][me_e] = psi_f(p.psi_form, rho0 + drho(n));
-    }
-    stage_ring(xs - 1, stage[0]);
-    __syncthreads();
-    reduce_ring(xs - 1, stage[0]);
-    sw_cur = solid_word(xs);
-    if (!(sw_cur >> 31)) {
-      pull_own(xs, sw_cur, cur);
-      rho_cur = rho0 + drho(cur);
-      psi_lds[pslot(xs)][f][me_e] = psi_f(p.psi_form, rho_cur);
-    }
-    stage_ring(xs, stage[1]);
-    __syncthreads();
-    reduce_ring(xs, stage[1]);
-    __syncthreads();
-    stage_ring(xs + 1, stage[(xs + 1) & 1]);
-    __syncthreads();
-  }
-
-  uint32_t sw_ahead = solid_word(xs + 1);
-  for (int x = xs; x < xe; ++x) {
-    float nxt[Q];
-    float rho_nxt = 0.0f;
-    const uint32_t sw_nxt = sw_ahead;
-    sw_ahead = solid_word(x + 2);
-    if (!(sw_nxt >> 31)) {
-      pull_own(x + 1, sw_nxt, nxt);
-      rho_nxt = rho0 + drho(nxt);
-      psi_lds[pslot(x + 1)][f][me_e] = psi_f(p.psi_form, rho_nxt);
-    }
-    reduce_ring(x + 1, stage[(x + 1) & 1]);
-    stage_ring(x + 2, stage[x & 1]);
-    __syncthreads();
-    {
-      float j[3];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) j[d] = 0.0f;
-#pragma unroll
-      for (int i = 0; i < Q; ++i)
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-          if (L::c[i][d] != 0) j[d] += (float)L::c[i][d] * cur[i];
-      const float rho_p = shfl_xor32f(rho_cur);
-      float j_p[3];
-      const bool bdist2 = p.eq_form == kBDist2;
-      if (bdist2) {
-#pragma unroll
-        for (int d = 0; d < 3; ++d) j_p[d] = shfl_xor32f(j[d]);
-      }
-      if (!(sw_cur >> 31)) {
-        // Shan-Chen force on component f (force.d:238-309)
-        float F[3] = {0.0f, 0.0f, 0.0f};
-        const float psi1 = psi_lds[pslot(x)][f][me_e];
-        if (p.sc) {
-#pragma unroll
-          for (int f2 = 0; f2 < 2; ++f2) {
-            const float cc = p.gcc[f][f2];
-            if (cc == 0.0f) continue;
-            const float psi_self = psi_lds[pslot(x)][f2][me_e];
-            float acc[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int i = 1; i < Q; ++i) {
-              const bool sn = SOLID && ((sw_cur >> i) & 1u);
-              const float psi2 =
-                  sn ? psi_self
-                     : psi_lds[pslot(x + L::c[i][0])][f2][me_e + L::c[i][1] * Sh::EZ + L::c[i][2]];
-              const float wp = (float)L::w[i] * psi2;
-#pragma unroll
-              for (int d = 0; d < 3; ++d)
-                if (L::c[i][d] != 0) acc[d] += (float)L::c[i][d] * wp;
-            }
-#pragma unroll
-            for (int d = 0; d < 3; ++d) F[d] -= psi1 * cc * acc[d];
-          }
-          const float wc = p.gwc[f];
-          if (SOLID && wc != 0.0f) {
-#pragma unroll
-            for (int i = 0; i < Q; ++i) {
-              if ((sw_cur >> i) & 1u) {
-                const int nb = gsite(wrapf(x + L::c[i][0], nx), wrapf(y + L::c[i][1], ny),
-                                     wrapf(z + L::c[i][2], nz));
-                const float pf = (float)L::w[i] * psi1 * (float)S[f * sp + nb] * wc;
-#pragma unroll
-                for (int d = 0; d < 3; ++d) F[d] -= pf * (float)L::c[i][d];
-              }
-            }
-          }
-        }
-        if (p.has_body) {
-#pragma unroll
-          for (int d = 0; d < 3; ++d) F[d] += p.body[f][d];
-        }
-        const float rinv = 1.0f / rho_cur;
-        float u[3];
-        if (bdist2) {
-          const float tot_inv = 1.0f / (rho_cur + rho_p);
-          const float tau_p = p.tau[f ^ 1];
-#pragma unroll
-          for (int d = 0; d < 3; ++d) {
-            const float wv = (j[d] / p.tau[f] + j_p[d] / tau_p) * tot_inv;
-            u[d] = p.tau[f] * (p.gacc[d] + F[d] * rinv) + wv;
-          }
-        } else {
-#pragma unroll
-          for (int d = 0; d < 3; ++d) u[d] = p.tau[f] * (p.gacc[d] + F[d] * rinv) + j[d] * rinv;
-        }
-        float e[Q];
-        eq_dev_d3q19(p.eq_form, u, e);
-        const float drh = rho_cur - rho0;
-        const float om = p.omega[f];
-        const int s = x * plane + y * nz + z;
-#pragma unroll
-        for (int i = 0; i < Q; ++i) {
-          const float deq = (float)L::w[i] * drh + rho_cur * e[i];
-          Bf[(int64_t)i * sp + s] = from_f<Store>(cur[i] + om * (deq - cur[i]));
-        }
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < Q; ++i) cur[i] = nxt[i];
-    rho_cur = rho_nxt;
-    sw_cur = sw_nxt;
-  }
-}
-
-template <class Store>
-inline int launch_shifted_d3q19(const void *A, void *B, const uint32_t *nbw, const double *S,
-                                const Geo &g, const ShiftParams &p, int target_blocks,
-                                hipStream_t stream) {
-  if (!fused_d3q19_supported(g)) return -1;
-  int chunk, nchunks, ty, tz;
-  // FP32 arithmetic needs about half the registers of the FP64 kernel: two
-  // workgroups per CU by default
-  fused_d3q19_grid(g, target_blocks > 0 ? target_blocks : 512, chunk, nchunks, ty, tz);
-  const int nblocks = ty * tz * nchunks;
-  constexpr int threads = 64 * kFusedTY;
-  const Store *a = static_cast<const Store *>(A);
-  Store *b = static_cast<Store *>(B);
-  if (p.has_solid)
-    hipLaunchKernelGGL((k_shifted_d3q19<Store, kFusedTY, true>), dim3(nblocks), dim3(threads), 0,
-                       stream, a, b, nbw, S, g, p, chunk, ty, tz, nchunks);
-  else
-    hipLaunchKernelGGL((k_shifted_d3q19<Store, kFusedTY, false>), dim3(nblocks), dim3(threads),
-                       0, stream, a, b, nbw, S, g, p, chunk, ty, tz, nchunks);
-  return 0;
-}
 
 }  // namespace dlbc

@@ -63,6 +63,15 @@ def lib():
         "dlbc_gpu_get_schedule": (i, [vp]),
         "dlbc_gpu_set_launch": (i, [vp, i]),
         "dlbc_gpu_laplace": (i, [vp, _dp]),
+        "dlbc_batch_create": (i, [C.POINTER(vp), C.POINTER(vp), i]),
+        "dlbc_batch_set_config": (i, [vp, i, i, i, i]),
+        "dlbc_batch_get_config": (i, [vp, C.POINTER(i)]),
+        "dlbc_batch_autotune": (i, [vp, C.POINTER(i), C.POINTER(d)]),
+        "dlbc_batch_step": (i, [vp, i, i]),
+        "dlbc_batch_step_looped": (i, [vp, i]),
+        "dlbc_batch_synchronize": (i, [vp]),
+        "dlbc_batch_last_error": (C.c_char_p, [vp]),
+        "dlbc_batch_destroy": (None, [vp]),
         "dlbc_gpu_download_populations": (i, [vp, i, _dp, i]),
         "dlbc_gpu_density": (i, [vp, i, _dp, i]),
         "dlbc_gpu_velocity": (i, [vp, i, _dp, i]),
@@ -341,3 +350,58 @@ class Lattice:
         for f in range(nc):
             L.upload_population(f, pops[f], halo=2)
         return L
+
+
+class Batch:
+    """Independent D3Q19 two-component lattices (shifted storage, any size)
+    advanced together: one persistent launch per step (dlbc_batch_*)."""
+
+    def __init__(self, lattices):
+        self.members = list(lattices)  # keep the contexts alive
+        arr = (C.c_void_p * len(self.members))(*[m.h for m in self.members])
+        h = C.c_void_p()
+        rc = lib().dlbc_batch_create(C.byref(h), arr, len(self.members))
+        if rc:
+            raise DlbcError("dlbc_batch_create: %s" % ERRORS.get(rc, rc))
+        self.h = h
+
+    def _check(self, rc, what):
+        if rc:
+            raise DlbcError("%s: %s (%s)" % (what, ERRORS.get(rc, rc),
+                                             lib().dlbc_batch_last_error(self.h).decode()))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().dlbc_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_config(self, tile_y, chunk, workgroups, kernel=0):
+        self._check(lib().dlbc_batch_set_config(self.h, tile_y, chunk, workgroups, kernel),
+                    "set_config")
+
+    @property
+    def config(self):
+        out = (C.c_int * 4)()
+        self._check(lib().dlbc_batch_get_config(self.h, out), "get_config")
+        return tuple(out)
+
+    def autotune(self):
+        out = (C.c_int * 4)()
+        ms = C.c_double()
+        self._check(lib().dlbc_batch_autotune(self.h, out, C.byref(ms)), "autotune")
+        return tuple(out), ms.value
+
+    def step(self, nsteps=1, graph=True):
+        self._check(lib().dlbc_batch_step(self.h, int(nsteps), int(bool(graph))), "step")
+
+    def step_looped(self, nsteps=1):
+        self._check(lib().dlbc_batch_step_looped(self.h, int(nsteps)), "step_looped")
+
+    def synchronize(self):
+        self._check(lib().dlbc_batch_synchronize(self.h), "synchronize")

@@ -184,6 +184,36 @@ int64_t dlbc_gpu_device_bytes(const dlbc_gpu_ctx *ctx);
  * order for the seeded kinds (std.random.Mt19937, uniform!"[)").
  * --------------------------------------------------------------------- */
 
+/* ------------------------------------------------------------------------
+ * Batches: many independent lattices advanced by one persistent launch per
+ * step.  No counterpart in the reference (one lattice per run); each member
+ * is an ordinary context (its own parameters, uploads, fields, reductions)
+ * created with DLBC_STORAGE_F32_SHIFT or F16_SHIFT, D3Q19, two components, of
+ * any size n >= 4.  The caller keeps the members alive while the batch
+ * exists.  Results are the same bits as stepping each member alone.
+ * ------------------------------------------------------------------------ */
+typedef struct dlbc_batch dlbc_batch;
+
+int dlbc_batch_create(dlbc_batch **out, dlbc_gpu_ctx *const *lattices, int n);
+/* Tile height (4 or 8 rows of 32 sites), x-chunk length (planes), number of
+ * workgroups of the persistent launch and kernel (0: solid-capable, 128
+ * VGPRs; 1: for batches without solid sites, more loads in flight).
+ * Defaults 8, whole lattice, 512, 0. */
+int dlbc_batch_set_config(dlbc_batch *b, int tile_y, int chunk, int workgroups, int kernel);
+int dlbc_batch_get_config(const dlbc_batch *b, int *out /*4*/);
+/* Time every (tile height, chunk, workgroups, kernel) candidate on the
+ * current state (reads it, writes only the buffer the next step overwrites)
+ * and keep the fastest; out receives it, best_ms its launch time. */
+int dlbc_batch_autotune(dlbc_batch *b, int *out /*4*/, double *best_ms);
+/* nsteps steps, one launch per step; with use_graph pairs of steps are
+ * replayed from a captured HIP graph. */
+int dlbc_batch_step(dlbc_batch *b, int nsteps, int use_graph);
+/* The same steps as one launch per member and step (the per-lattice loop). */
+int dlbc_batch_step_looped(dlbc_batch *b, int nsteps);
+int dlbc_batch_synchronize(dlbc_batch *b);
+const char *dlbc_batch_last_error(const dlbc_batch *b);
+void dlbc_batch_destroy(dlbc_batch *b);
+
 /* FluidInit (src/dlbc/lb/init.d:139-260). */
 enum {
   DLBC_INIT_NONE = 0, DLBC_INIT_FILE, DLBC_INIT_CONST, DLBC_INIT_EQDIST,

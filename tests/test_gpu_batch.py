@@ -1,0 +1,135 @@
+"""Batches of lattices (one persistent launch per step over many lattices)
+and the tiled shifted-storage kernel on lattices of any size.
+
+A lattice stepped inside a batch must give the same bits as the same
+lattice stepped alone, whatever the tile height, chunk length, workgroup
+count, graph replay or per-lattice loop; the tiled kernel on unaligned
+lattices is compared with the FP64 path (itself bit-identical to the oracle)
+within the FP32 bounds of test_gpu_shifted.py.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def D():
+    import dlbc_amd
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("no GPU visible")
+    return dlbc_amd
+
+
+def droplet(n, tau, g, frac):
+    return dict(d=3, q=19, gn=list(n), components=2, tau=[tau, tau], eqDistForm="BDist2",
+                gcc=[[0.0, g], [g, 0.0]], enableShanChen=True, psiForm="Exponential",
+                fluidInit=["EqDistSphereFrac"] * 2, fluidDensities=[[0.7, 0.04], [0.04, 0.7]],
+                initRadius=frac, interfaceThickness=5.0, rho0=[0.37, 0.37])
+
+
+# uneven shapes: partial tiles in y and z, odd lengths, one aligned lattice
+SHAPES = [((33, 40, 37), 1.0, 3.6, 0.25), ((20, 36, 52), 1.3, 4.2, 0.3),
+          ((32, 32, 64), 1.1, 3.9, 0.2), ((17, 9, 70), 1.45, 4.5, 0.33)]
+
+
+def _members(D, storage="f32_shift"):
+    return [D.Lattice.from_params(droplet(*s), storage=storage) for s in SHAPES]
+
+
+def _pops(lats):
+    return [[L.population(f) for f in range(2)] for L in lats]
+
+
+def _same(a, b, what):
+    for k, (pa, pb) in enumerate(zip(a, b)):
+        for f in range(2):
+            assert np.array_equal(pa[f].view(np.uint64), pb[f].view(np.uint64)), (what, k, f)
+
+
+@pytest.mark.parametrize("storage", ["f32_shift", "f16_shift"])
+def test_batch_equals_lattices_stepped_alone(D, storage):
+    """11 steps (odd: the buffers end swapped) in one batch, graph replay on,
+    equal bit for bit to stepping every lattice alone; the per-lattice loop
+    gives the same bits too."""
+    alone = _members(D, storage)
+    for L in alone:
+        L.step(11)
+    ref = _pops(alone)
+    lats = _members(D, storage)
+    B = D.Batch(lats)
+    B.step(11, graph=True)
+    _same(_pops(lats), ref, "batch")
+    lats2 = _members(D, storage)
+    D.Batch(lats2).step_looped(11)
+    _same(_pops(lats2), ref, "looped")
+
+
+def test_batch_graph_replays(D):
+    """The captured two-step graph replayed several times, with eager work
+    (field downloads, an eager step, set_params) between the replays: every
+    replay matches the same sequence run eagerly."""
+    a, b = _members(D), _members(D)
+    Ba, Bb = D.Batch(a), D.Batch(b)
+    for k in range(4):
+        Ba.step(4, graph=True)
+        Bb.step(4, graph=False)
+        _same(_pops(a), _pops(b), "replay %d" % k)
+        for L in a + b:  # eager work between the replays
+            L.density(0)
+        Ba.step(1, graph=True)
+        Bb.step(1, graph=False)
+        _same(_pops(a), _pops(b), "odd step %d" % k)
+    a[1].set_params([1.2, 1.2], [[0.0, 4.0], [4.0, 0.0]], None, None, None, "BDist2",
+                    "Exponential", True, [0.37, 0.37])
+    b[1].set_params([1.2, 1.2], [[0.0, 4.0], [4.0, 0.0]], None, None, None, "BDist2",
+                    "Exponential", True, [0.37, 0.37])
+    Ba.step(6, graph=True)
+    Bb.step(6, graph=False)
+    _same(_pops(a), _pops(b), "after set_params")
+
+
+def test_batch_config_and_autotune_do_not_change_results(D):
+    ref = None
+    for cfg in [(8, 1 << 20, 512, 0), (4, 16, 64, 1), (8, 8, 2048, 1), (4, 32, 8, 0)]:
+        lats = _members(D)
+        B = D.Batch(lats)
+        B.set_config(*cfg)
+        B.step(5)
+        got = _pops(lats)
+        if ref is None:
+            ref = got
+        else:
+            _same(got, ref, "config %s" % (cfg,))
+    lats = _members(D)
+    B = D.Batch(lats)
+    cfg, ms = B.autotune()
+    assert cfg[0] in (4, 8) and ms > 0
+    B.step(5)
+    _same(_pops(lats), ref, "autotuned %s" % (cfg,))
+
+
+def test_unaligned_shifted_lattice_close_to_fp64(D):
+    """A lattice whose y and z lengths are not multiples of the tile (partial
+    tiles on both axes) against the FP64 path: the same bounds as the
+    aligned FP32 droplet tests."""
+    p = droplet((44, 42, 50), 1.0, 3.6, 0.25)
+    ref = D.Lattice.from_params(p)
+    L = D.Lattice.from_params(p, storage="f32_shift")
+    m0 = [L.total_mass(f) for f in range(2)]
+    ref.step(500)
+    L.step(500)
+    for f in range(2):
+        assert np.abs(L.density(f) - ref.density(f)).max() <= 3e-5
+        assert np.abs(L.velocity(f) - ref.velocity(f)).max() <= 1e-5
+        assert abs(L.total_mass(f) - m0[f]) <= 1e-4 * m0[f]
+
+
+def test_batch_refuses_mixed_members(D):
+    a = D.Lattice.from_params(droplet((16, 16, 32), 1.0, 3.6, 0.25), storage="f32_shift")
+    b = D.Lattice.from_params(droplet((16, 16, 32), 1.0, 3.6, 0.25), storage="f16_shift")
+    c = D.Lattice.from_params(droplet((16, 16, 32), 1.0, 3.6, 0.25))
+    for members in ([a, b], [a, c], [a, a]):
+        with pytest.raises(D.DlbcError):
+            D.Batch(members)

@@ -120,11 +120,11 @@ def test_exact_constant_division_selftest():
 
 
 @pytest.mark.parametrize("d,q,nc,n", [(2, 9, 2, [64, 64]), (3, 19, 1, [8, 8, 32]),
-                                      (3, 19, 2, [8, 12, 32]), (3, 19, 2, [8, 8, 48])])
+                                      (3, 19, 2, [8, 3, 32]), (3, 19, 2, [8, 8, 2])])
 def test_shifted_storage_shape_checks(d, q, nc, n):
-    """The shifted storages exist for the fused D3Q19 two-component kernel
-    only (n_y % 8 == 0, n_z % 32 == 0); other shapes are refused before any
-    device call."""
+    """The shifted storages exist for the tiled D3Q19 two-component kernel
+    (any n >= 4 per axis); other lattices are refused before any device
+    call."""
     import dlbc_amd as D
     with pytest.raises(D.DlbcError, match="unsupported"):
         D.Lattice(d, q, nc, n, storage="f32_shift")
