@@ -328,7 +328,7 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--schedule", default="auto", choices=["auto", "two_pass", "fused"])
+    ap.add_argument("--schedule", default="auto", choices=["auto", "two_pass", "fused", "split"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-storage-legs", action="store_true")
     ap.add_argument("--workload", default="channel", choices=["channel", "droplet", "ensemble"],

@@ -14,6 +14,7 @@
 #include "kernels_batch.hpp"
 #include "kernels_fused.hpp"
 #include "kernels_shifted.hpp"
+#include "kernels_split.hpp"
 #include "kernels_twopass.hpp"
 
 using namespace dlbc;
@@ -439,6 +440,30 @@ struct FusedStep {
   }
 };
 
+// Lane-per-component two-pass schedule (D3Q19, two components, FP64).
+template <class L, int C>
+struct SplitStep {
+  static int run(dlbc_gpu_ctx *c, Timer *tm, double *acc_ms) {
+    (void)acc_ms;
+    if constexpr (std::is_same<L, D3Q19>::value && C == 2) {
+      const double *A = static_cast<const double *>(c->popA);
+      double *B = static_cast<double *>(c->popB);
+      if (c->p.sc) {
+        tm->before();
+        launch_split_d3q19(A, B, c->nbw, c->S, c->geo, c->p, c->stream, 0);
+        tm->after();
+      }
+      tm->before();
+      launch_split_d3q19(A, B, c->nbw, c->S, c->geo, c->p, c->stream, 1);
+      tm->after();
+      std::swap(c->popA, c->popB);
+      return DLBC_OK;
+    } else {
+      return fail(c, DLBC_ERR_UNSUPPORTED, "split schedule needs D3Q19 with 2 components");
+    }
+  }
+};
+
 template <class L, int C>
 struct AdvectOnly {
   static int run(dlbc_gpu_ctx *c) {
@@ -505,6 +530,8 @@ int do_steps(dlbc_gpu_ctx *c, int nsteps, Timer *tm) {
     tm->slot = 0;
     if (sched == DLBC_SCHED_FUSED)
       rc = dispatch<FusedStep>(c, tm, (double *)nullptr);
+    else if (sched == DLBC_SCHED_SPLIT)
+      rc = dispatch<SplitStep>(c, tm, (double *)nullptr);
     else
       rc = dispatch<TwoPassStep>(c, tm, (double *)nullptr);
     if (rc) return rc;
@@ -934,6 +961,10 @@ const char *dlbc_gpu_kernel_name(const dlbc_gpu_ctx *c, int i) {
   if (!c) return "";
   const int sched = effective_schedule(c);
   if (sched == DLBC_SCHED_FUSED) return i == 0 ? "fused_d3q19" : "";
+  if (sched == DLBC_SCHED_SPLIT) {
+    if (c->p.sc) return i == 0 ? "psi_split" : (i == 1 ? "stream_force_collide_split" : "");
+    return i == 0 ? "stream_collide_split" : "";
+  }
   if (c->p.sc) return i == 0 ? "psi" : (i == 1 ? "stream_force_collide" : "");
   return i == 0 ? "stream_collide" : "";
 }
@@ -945,7 +976,11 @@ int dlbc_gpu_synchronize(dlbc_gpu_ctx *c) {
 }
 
 int dlbc_gpu_set_schedule(dlbc_gpu_ctx *c, int schedule) {
-  if (!c || schedule < 0 || schedule > 2) return DLBC_ERR_ARG;
+  if (!c || schedule < 0 || schedule > 3) return DLBC_ERR_ARG;
+  if (schedule == DLBC_SCHED_SPLIT && !(c->d == 3 && c->C == 2))
+    return fail(c, DLBC_ERR_UNSUPPORTED, "split schedule needs D3Q19 with 2 components");
+  if (schedule == DLBC_SCHED_SPLIT && c->storage != DLBC_STORAGE_F64)
+    return fail(c, DLBC_ERR_UNSUPPORTED, "the split schedule needs FP64 storage");
   if (schedule == DLBC_SCHED_FUSED && c->storage == DLBC_STORAGE_F64 &&
       !(c->d == 3 && c->C == 2 && fused_d3q19_supported(c->geo)))
     return fail(c, DLBC_ERR_UNSUPPORTED,

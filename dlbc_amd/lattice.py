@@ -18,7 +18,7 @@ FLUID_INIT = {n: i for i, n in enumerate([
 MASK_INIT = {"None": 0, "File": 1, "Plates": 2, "Tube": 3, "Box": 4, "Walls": 5}
 AXIS = {"X": 0, "Y": 1, "Z": 2}
 STORAGE = {"f64": 0, "f32_shift": 1, "f16_shift": 2}
-SCHEDULES = {"auto": 0, "two_pass": 1, "fused": 2}
+SCHEDULES = {"auto": 0, "two_pass": 1, "fused": 2, "split": 3}
 ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "out of memory",
           -4: "unsupported configuration", -5: "call out of order"}
 
@@ -267,7 +267,7 @@ class Lattice:
     @property
     def schedule(self):
         v = lib().dlbc_gpu_get_schedule(self.h)
-        return {0: "auto", 1: "two_pass", 2: "fused"}[v]
+        return {0: "auto", 1: "two_pass", 2: "fused", 3: "split"}[v]
 
     def population(self, f, halo=0):
         out = np.zeros(_padded_shape(self.n, halo) + (self.q,))

@@ -63,7 +63,8 @@ enum { DLBC_PSI_LINEAR = 0, DLBC_PSI_EXPONENTIAL = 1 };
 enum {
   DLBC_SCHED_AUTO = 0,     /* fastest available for the configuration        */
   DLBC_SCHED_TWO_PASS = 1, /* psi pass + stream/force/collide pass (any DdQq)  */
-  DLBC_SCHED_FUSED = 2     /* single-pass D3Q19 kernel with an LDS psi ring   */
+  DLBC_SCHED_FUSED = 2,    /* single-pass D3Q19 kernel with an LDS psi ring   */
+  DLBC_SCHED_SPLIT = 3     /* D3Q19 two-component two-pass, lane per component */
 };
 
 /* Create a context for one lattice on HIP device `device`.

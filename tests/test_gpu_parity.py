@@ -316,16 +316,33 @@ def test_fused_schedule_matches_oracle(D, oracle_mod, form, walls):
 
 
 def test_fused_equals_two_pass_with_body_force(D):
-    """Both schedules give the same bits, body force included, over several
-    chunks of x and tiles of (y, z)."""
+    """All three schedules give the same bits, body force included, over
+    several chunks of x and tiles of (y, z)."""
     p = dict(_walls_case([24, 32, 64], "SecondOrder"),
              bodyForce=[[0.0, 0.0, 0.0], [1e-6, 0.0, 0.0]])
     a = D.Lattice.from_params(p, schedule="fused")
     b = D.Lattice.from_params(p, schedule="two_pass")
-    a.step(60)
-    b.step(60)
+    c = D.Lattice.from_params(p, schedule="split")
+    for L in (a, b, c):
+        L.step(60)
     for f in range(2):
         assert_close(a.population(f), b.population(f), 0.0, "fused vs two-pass f%d" % f)
+        assert_close(c.population(f), b.population(f), 0.0, "split vs two-pass f%d" % f)
+
+
+@pytest.mark.parametrize("form", ["SecondOrder", "BDist2", "ThirdOrder"])
+@pytest.mark.parametrize("walls", [False, True], ids=["periodic", "walls"])
+def test_split_schedule_matches_oracle(D, oracle_mod, form, walls):
+    """The lane-per-component two-pass D3Q19 kernels against the oracle, bit
+    for bit, on a lattice no tile divides (any shape is allowed)."""
+    n = [9, 13, 21]
+    p = _walls_case(n, form) if walls else dict(_config2(8, form), gn=n)
+    g = D.Lattice.from_params(p, schedule="split")
+    assert g.schedule == "split"
+    o = oracle_mod.Oracle.from_params(p)
+    g.step(40)
+    o.run(40)
+    _same_state(g, o, "split %s walls=%s" % (form, walls))
 
 
 def _trace_close(got, row, rel=6e-7):
