@@ -86,17 +86,18 @@ def build_inputs(D, p, mask):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "step_kernels_channel256_traffic.txt")
 
 
-def pmc_traffic(storage, n):
-    """HBM bytes per launch of the step kernel on this workload, from the
+def pmc_traffic(storage, kernel, n):
+    """HBM bytes per launch of a step kernel on this workload, from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters scaled by a
-    copy kernel of known size); None for sizes that summary does not cover."""
+    copy kernel of known size); None where that summary has no figure."""
     if n != 256 or not os.path.exists(TRAFFIC_FILE):
         return None
     with open(TRAFFIC_FILE) as fh:
         for line in fh:
             parts = line.split()
-            if len(parts) >= 6 and parts[0] == "traffic" and parts[1] == storage:
-                return int(parts[5])
+            if len(parts) >= 7 and parts[0] == "traffic" and parts[1] == storage \
+                    and parts[2] == kernel:
+                return int(parts[6])
     return None
 
 
@@ -115,6 +116,16 @@ def make_lattice(D, p, mask, pops, schedule, storage="f64", rho0=None):
 def alg_bytes(fluid, sites, elem):
     """Populations read and written once per fluid site, one mask byte per site."""
     return fluid * 2 * C * Q * elem + sites
+
+
+def kernel_bytes(name, fluid, sites, elem):
+    """Algorithmic bytes of one launch of a step kernel: the populations it
+    must read and write once, psi written by a psi pass, one mask byte per
+    site for a kernel that streams."""
+    pops = fluid * C * Q * elem
+    if name.startswith("psi"):
+        return pops + fluid * C * 8  # read populations, write psi
+    return 2 * pops + sites
 
 
 def time_steps(L, steps, warmup):
@@ -146,7 +157,7 @@ def storage_leg(D, p, mask, pops, storage, steps, warmup, fluid):
            "launch_ms": round(kern[dom], 4), "bytes_per_launch": b,
            "achieved": round(b / (kern[dom] * 1e-3) / 1e9, 1),
            "frac": round(b / (kern[dom] * 1e-3) / HBM_PEAK, 4),
-           "traffic": pmc_traffic(storage, p["gn"][0]), "rho0": rho0,
+           "traffic": pmc_traffic(storage, dom, p["gn"][0]), "rho0": rho0,
            "mass_rel_drift": float("%.3g" % drift)}
     del L
     return out
@@ -373,8 +384,11 @@ def main():
     el, kern = L.step_timed(max(5, args.steps // 2))
     dom = max(kern, key=kern.get)
     fluid = int(sites - mask.sum())
-    b_alg = alg_bytes(fluid, sites, 8)
+    b_alg = kernel_bytes(dom, fluid, sites, 8)
     achieved = b_alg / (kern[dom] * 1e-3)
+    per_kernel = {k: {"launch_ms": round(v, 4), "bytes": kernel_bytes(k, fluid, sites, 8),
+                      "GBps": round(kernel_bytes(k, fluid, sites, 8) / (v * 1e-3) / 1e9, 1)}
+                  for k, v in kern.items()}
     log("kernels (ms/launch): %s" % kern)
     m1 = [L.total_mass(f) for f in range(2)]
     drift = max(abs(m1[f] - m0[f]) / m0[f] for f in range(2))
@@ -395,9 +409,11 @@ def main():
                    "mass_rel_drift": float("%.3g" % drift)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 1),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK, 4), "traffic": pmc_traffic("f64", n),
+                     "frac": round(achieved / HBM_PEAK, 4), "traffic": pmc_traffic("f64", dom, n),
                      "bytes_per_launch": b_alg, "launch_ms": round(kern[dom], 4),
-                     "step_frac": round(mlups * 1e6 * (b_alg / sites) / HBM_PEAK, 4)},
+                     "step_frac": round(mlups * 1e6 * (alg_bytes(fluid, sites, 8) / sites)
+                                        / HBM_PEAK, 4),
+                     "kernels": per_kernel},
     }
     del L
     torch.cuda.empty_cache()

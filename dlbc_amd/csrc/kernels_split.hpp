@@ -3,11 +3,13 @@
 //
 // The general two-pass schedule (kernels_twopass.hpp) gives a thread a whole
 // site: both components' populations (76 doubles) live in its registers and
-// every neighbour's solid flag is a separate mask load.  Here the two
-// components of a site sit on partner lanes l and l ^ 32 of one wave (as in
-// the fused kernel), which halves the registers per thread, and the solid
-// neighbourhood is the site's one 32-bit solid word.  The new populations are
-// written with streaming stores.
+// every neighbour's solid flag is a separate mask load.  Here a wave holds one
+// component of 64 consecutive sites (waves 2k and 2k+1 of a block: the two
+// components of the same sites), which halves the registers per thread and
+// makes every array base wave-uniform, so loads and stores address with a
+// scalar base and a 32-bit offset.  The partner component's density and
+// momentum pass through LDS.  The solid neighbourhood is the site's one
+// 32-bit solid word; the new populations are written with streaming stores.
 //
 //   pass 1: pull the 19 streamed populations, density, psi -> S
 //   pass 2: pull again, Shan-Chen force from psi in S, velocity shift,
@@ -24,17 +26,36 @@ namespace dlbc {
 
 constexpr int kSplitThreads = 256;  // 4 waves = 128 sites x 2 components
 
-// Site and component of this lane; false past the end of the lattice.
+// Site and component of this lane (the component is uniform per wave); false
+// past the end of the lattice.
 __device__ inline bool split_lane(const Geo &g, int &s, int &f) {
-  const int lane = threadIdx.x & 63;
-  f = lane >> 5;
-  s = blockIdx.x * (kSplitThreads / 2) + (threadIdx.x >> 6) * 32 + (lane & 31);
+  const int w = threadIdx.x >> 6;
+  f = w & 1;
+  s = blockIdx.x * (kSplitThreads / 2) + (w >> 1) * 64 + (threadIdx.x & 63);
   return s < g.sites;
 }
 
-// The 19 streamed populations of component f at site s = (x, y, z) whose
-// solid word is sw (pull with bounce-back, src/dlbc/lb/advection.d:61-89).
-template <bool SOLID>
+// Element k of the array at `base` (wave-uniform).  OFF32 (every index of
+// the kernel below 2^29): the byte offset fits 32 bits, so the access takes a
+// scalar base and a 32-bit lane offset instead of a 64-bit lane address.
+template <bool OFF32>
+__device__ inline double *elem(double *base, int64_t k) {
+  if constexpr (OFF32)
+    return reinterpret_cast<double *>(reinterpret_cast<char *>(base) + ((uint32_t)k << 3));
+  else
+    return base + k;
+}
+template <bool OFF32>
+__device__ inline const double &at(const double *base, int64_t k) {
+  return *elem<OFF32>(const_cast<double *>(base), k);
+}
+
+// True when every population index of a lattice (19 sp) is below 2^29.
+inline bool split_off32(const Geo &g) { return 19 * g.sp < ((int64_t)1 << 29); }
+
+// The 19 streamed populations of component f (array Af) at site s = (x, y, z)
+// whose solid word is sw (pull with bounce-back, src/dlbc/lb/advection.d:61-89).
+template <bool SOLID, bool OFF32>
 __device__ inline void split_pull(const double *__restrict__ Af, const Geo &g, int s, int x,
                                   int y, int z, uint32_t sw, double *n) {
   using L = D3Q19;
@@ -45,11 +66,11 @@ __device__ inline void split_pull(const double *__restrict__ Af, const Geo &g, i
                        : site_of(wrap(x - L::c[i][0], g.nx), wrap(y - L::c[i][1], g.ny),
                                  wrap(z - L::c[i][2], g.nz), g);
     const int comp = bb ? L::bounce[i] : i;
-    n[i] = Af[(int64_t)comp * g.sp + src];
+    n[i] = at<OFF32>(Af, (int64_t)comp * g.sp + src);
   }
 }
 
-template <bool SOLID>
+template <bool SOLID, bool OFF32>
 __global__ void __launch_bounds__(kSplitThreads)
     k_split_psi(const double *__restrict__ A, const uint32_t *__restrict__ nbw,
                 double *__restrict__ S, Geo g, int psi_form) {
@@ -60,73 +81,238 @@ __global__ void __launch_bounds__(kSplitThreads)
   int x, y, z;
   coords(s, g, x, y, z);
   double n[19];
-  split_pull<SOLID>(A + (int64_t)f * 19 * g.sp, g, s, x, y, z, sw, n);
-  S[f * g.sp + s] = psi_of(psi_form, density<D3Q19>(n));
+  split_pull<SOLID, OFF32>(A + (int64_t)f * 19 * g.sp, g, s, x, y, z, sw, n);
+  *elem<OFF32>(S, (int64_t)f * g.sp + s) = psi_of(psi_form, density<D3Q19>(n));
 }
 
-template <bool SOLID>
+// addShanChenForcePsi (src/dlbc/lb/force.d:238-309) for two components whose
+// self-couplings gcc[f][f] are zero: the reference skips the f2 == f1 term,
+// so only the other component contributes and every lane of a wave runs the
+// same loop (sc_force with its lane-dependent skip runs both halves masked).
+// Same operations in the same order as sc_force, hence the same bits.
+template <class Psi, class Solid, class RhoSolid>
+__device__ inline void sc_force_offdiag(int f1, double (&F)[3], const StepParams &p, Psi psi,
+                                        Solid solid, RhoSolid rho_solid) {
+  using L = D3Q19;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) F[d] = 0.0;
+  if (!p.sc) return;
+  const double psi1 = psi(f1, -1);
+  const int f2 = f1 ^ 1;
+  const double cc = p.gcc[f1][f2];
+  if (cc != 0.0) {
+#pragma unroll
+    for (int i = 1; i < L::q; ++i) {
+      const double psi2 = solid(i) ? psi(f2, -1) : psi(f2, i);
+      const double pf = L::w[i] * psi1 * psi2 * cc;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) F[d] -= pf * static_cast<double>(L::c[i][d]);
+    }
+  }
+  const double wc = p.gwc[f1];
+  if (wc != 0.0 && p.has_solid) {
+#pragma unroll
+    for (int i = 0; i < L::q; ++i) {
+      if (solid(i)) {
+        const double pf = L::w[i] * psi1 * rho_solid(f1, i) * wc;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) F[d] -= pf * static_cast<double>(L::c[i][d]);
+      }
+    }
+  }
+}
+
+// collide_component (distributeForce, body force, velocity shift, eqDist and
+// BGK relaxation) for D3Q19 with every equilibrium value stored as soon as it
+// is formed, instead of through a 19-value array: the same operations in the
+// same order (lattice.hpp eq_dist, kernels_twopass.hpp collide_component), so
+// the same bits, with fewer registers live at the stores.
+template <class Out>
+__device__ inline void collide_d3q19_streamed(int f, const double *n, double rho, double total,
+                                              double (&F)[3], const double *wv,
+                                              const StepParams &p, Out out) {
+  using L = D3Q19;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) F[d] += 0.0 * (rho / total);
+  if (p.has_body) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) F[d] += p.body[f][d];
+  }
+  double dv[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    dv[d] = p.tau[f] * (p.gacc[d] + F[d] / rho);
+    if (p.eq_form == kBDist2) dv[d] += wv[d];
+  }
+  const double omw = p.one_minus_omega[f], om = p.omega[f];
+  auto relax = [&](int i, double eq) { out(i, n[i] * omw + om * eq); };
+  if (p.eq_form != kBDist2) {  // eqdist.d:82-100
+    double pv[3], v[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) pv[j] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 19; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) pv[j] += n[i] * static_cast<double>(L::c[i][j]);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) pv[j] /= rho;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) v[j] = dv[j] + pv[j];
+    double vdotv = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) vdotv += v[j] * v[j];
+    constexpr double css = kCss;
+    constexpr double css2 = 2.0 * css;
+    constexpr double css2p2 = css2 * css;
+    constexpr double css6p3 = 3.0 * css2p2 * css;
+    constexpr double r_css = 1.0 / css, r_css2 = 1.0 / css2, r_css2p2 = 1.0 / css2p2,
+                     r_css6p3 = 1.0 / css6p3;
+    const double vv_css2 = div_const(vdotv, css2, r_css2);
+    const bool third = p.eq_form == kThirdOrder;
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+      double vc = 0.0;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) vc += v[j] * static_cast<double>(L::c[i][j]);
+      double eq;
+      if (!third)
+        eq = rho * L::w[i] *
+             (1.0 + div_const(vc, css, r_css) + div_const(vc * vc, css2p2, r_css2p2) - vv_css2);
+      else
+        eq = rho * L::w[i] *
+             (1.0 + div_const(vc, css, r_css) + div_const(vc * vc, css2p2, r_css2p2) - vv_css2 +
+              div_const(vc * vc * vc, css6p3, r_css6p3) - div_const(vdotv * vc, css2p2, r_css2p2));
+      relax(i, eq);
+    }
+    return;
+  }
+  // BDist2, eqdist.d:102-184
+  const double aN1 = 1.0 / 36.0, aN0 = 1.0 / 3.0;
+  const double haN1 = 0.5 * aN1, aN1_6 = aN1 / 6.0;
+  const double T = 1.0 / 3.0, holdit = 1.0 / T;
+  const double tux = dv[0], tuy = dv[1], tuz = dv[2];
+  const double tuxt = tux * holdit, tuyt = tuy * holdit, tuzt = tuz * holdit;
+  const double uke = 0.5 * (tux * tuxt + tuy * tuyt + tuz * tuzt);
+  const double cst = 1.0 - uke;
+  const double const1 = aN1 * cst;
+  auto pair = [&](double cdotu, int ip, int im, bool axis) {
+    const double even = const1 + haN1 * cdotu * cdotu;
+    const double odd = const1 * cdotu + aN1_6 * cdotu * cdotu * cdotu;
+    if (axis) {
+      relax(ip, 2.0 * (even + odd) * rho);
+      relax(im, 2.0 * (even - odd) * rho);
+    } else {
+      relax(ip, (even + odd) * rho);
+      relax(im, (even - odd) * rho);
+    }
+  };
+  relax(0, aN0 * cst * rho);
+  pair(tuxt, 1, 2, true);
+  pair(tuyt, 3, 4, true);
+  pair(tuzt, 5, 6, true);
+  pair(tuxt + tuyt, 7, 14, false);
+  pair(tuxt - tuyt, 13, 8, false);
+  pair(tuxt + tuzt, 11, 18, false);
+  pair(tuxt - tuzt, 12, 17, false);
+  pair(tuyt + tuzt, 9, 16, false);
+  pair(tuyt - tuzt, 15, 10, false);
+}
+
+template <bool SOLID, bool OFFDIAG, bool OFF32>
 __global__ void __launch_bounds__(kSplitThreads)
     k_split_collide(const double *__restrict__ A, double *__restrict__ B,
                     const uint32_t *__restrict__ nbw, const double *__restrict__ S, Geo g,
                     StepParams p) {
   using L = D3Q19;
   constexpr int C = 2;
+  // density and first moment of both components of the block's sites
+  __shared__ double xch[2][4][kSplitThreads / 2];
   int s, f;
-  if (!split_lane(g, s, f)) return;
-  const uint32_t sw = SOLID ? nbw[s] : 0u;
-  if (sw >> 31) return;  // solid populations stay as they are
-  int x, y, z;
-  coords(s, g, x, y, z);
+  const bool inside = split_lane(g, s, f);
+  const int k = ((threadIdx.x >> 7) << 6) | (threadIdx.x & 63);  // site within the block
+  const uint32_t sw = !inside ? (1u << 31) : (SOLID ? nbw[s] : 0u);
+  int x = 0, y = 0, z = 0;
   double n[19];
-  split_pull<SOLID>(A + (int64_t)f * 19 * g.sp, g, s, x, y, z, sw, n);
-  const double rho = density<L>(n);
-  const double rho_partner = shfl_xor32(rho);
-  double mom[3], mom_p[3];
+  double rho = 0.0, mom[3] = {0.0, 0.0, 0.0};
   const bool bdist2 = p.eq_form == kBDist2;
+  if (!(sw >> 31)) {
+    coords(s, g, x, y, z);
+    split_pull<SOLID, OFF32>(A + (int64_t)f * 19 * g.sp, g, s, x, y, z, sw, n);
+    rho = density<L>(n);
+    if (bdist2) first_moment<L>(n, mom);
+  }
+  xch[f][0][k] = rho;
   if (bdist2) {
-    first_moment<L>(n, mom);
 #pragma unroll
-    for (int d = 0; d < 3; ++d) mom_p[d] = shfl_xor32(mom[d]);
+    for (int d = 0; d < 3; ++d) xch[f][1 + d][k] = mom[d];
+  }
+  __syncthreads();
+  if (sw >> 31) return;  // solid populations stay as they are
+  const double rho_partner = xch[f ^ 1][0][k];
+  double mom_p[3];
+  if (bdist2) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) mom_p[d] = xch[f ^ 1][1 + d][k];
   }
   auto nbr = [&](int i) {
     return site_of(wrap(x + L::c[i][0], g.nx), wrap(y + L::c[i][1], g.ny),
                    wrap(z + L::c[i][2], g.nz), g);
   };
   double F[3];
-  sc_force<L, C>(
-      f, F, p, [&](int ff, int i) { return S[ff * g.sp + (i < 0 ? s : nbr(i))]; },
-      [&](int i) { return SOLID && ((sw >> i) & 1u); },
-      [&](int ff, int i) { return S[ff * g.sp + nbr(i)]; });
+  auto psi = [&](int ff, int i) { return at<OFF32>(S, ff * g.sp + (i < 0 ? s : nbr(i))); };
+  auto solid = [&](int i) { return SOLID && ((sw >> i) & 1u); };
+  auto rho_solid = [&](int ff, int i) { return at<OFF32>(S, ff * g.sp + nbr(i)); };
+  if constexpr (OFFDIAG)
+    sc_force_offdiag(f, F, p, psi, solid, rho_solid);
+  else
+    sc_force<L, C>(f, F, p, psi, solid, rho_solid);
   const double total = total_density<C>([&](int ff) { return ff == f ? rho : rho_partner; });
   double wv[3];
   if (bdist2)
     weighted_velocity<L, C>([&](int ff, int d) { return ff == f ? mom[d] : mom_p[d]; }, total,
                             p, wv);
   double *__restrict__ Bf = B + (int64_t)f * 19 * g.sp;
-  collide_component<L>(f, n, rho, total, F, wv, p, [&](int i, double v) {
-    __builtin_nontemporal_store(v, &Bf[(int64_t)i * g.sp + s]);
+  collide_d3q19_streamed(f, n, rho, total, F, wv, p, [&](int i, double v) {
+    __builtin_nontemporal_store(v, elem<OFF32>(Bf, (int64_t)i * g.sp + s));
   });
+}
+
+template <bool OFF32>
+inline void launch_split_impl(const double *A, double *B, const uint32_t *nbw, double *S,
+                              const Geo &g, const StepParams &p, hipStream_t stream, int pass) {
+  const int nb = (g.sites + kSplitThreads / 2 - 1) / (kSplitThreads / 2);
+  const dim3 grid(nb), block(kSplitThreads);
+  if (pass == 0) {
+    if (p.has_solid)
+      hipLaunchKernelGGL((k_split_psi<true, OFF32>), grid, block, 0, stream, A, nbw, S, g,
+                         p.psi_form);
+    else
+      hipLaunchKernelGGL((k_split_psi<false, OFF32>), grid, block, 0, stream, A, nbw, S, g,
+                         p.psi_form);
+    return;
+  }
+  const bool offdiag = p.gcc[0][0] == 0.0 && p.gcc[1][1] == 0.0;
+  if (p.has_solid && offdiag)
+    hipLaunchKernelGGL((k_split_collide<true, true, OFF32>), grid, block, 0, stream, A, B, nbw,
+                       S, g, p);
+  else if (p.has_solid)
+    hipLaunchKernelGGL((k_split_collide<true, false, OFF32>), grid, block, 0, stream, A, B, nbw,
+                       S, g, p);
+  else if (offdiag)
+    hipLaunchKernelGGL((k_split_collide<false, true, OFF32>), grid, block, 0, stream, A, B, nbw,
+                       S, g, p);
+  else
+    hipLaunchKernelGGL((k_split_collide<false, false, OFF32>), grid, block, 0, stream, A, B,
+                       nbw, S, g, p);
 }
 
 inline void launch_split_d3q19(const double *A, double *B, const uint32_t *nbw, double *S,
                                const Geo &g, const StepParams &p, hipStream_t stream,
                                int pass) {
-  const int nb = (g.sites + kSplitThreads / 2 - 1) / (kSplitThreads / 2);
-  if (pass == 0) {
-    if (p.has_solid)
-      hipLaunchKernelGGL(k_split_psi<true>, dim3(nb), dim3(kSplitThreads), 0, stream, A, nbw, S,
-                         g, p.psi_form);
-    else
-      hipLaunchKernelGGL(k_split_psi<false>, dim3(nb), dim3(kSplitThreads), 0, stream, A, nbw,
-                         S, g, p.psi_form);
-  } else {
-    if (p.has_solid)
-      hipLaunchKernelGGL(k_split_collide<true>, dim3(nb), dim3(kSplitThreads), 0, stream, A, B,
-                         nbw, S, g, p);
-    else
-      hipLaunchKernelGGL(k_split_collide<false>, dim3(nb), dim3(kSplitThreads), 0, stream, A, B,
-                         nbw, S, g, p);
-  }
+  if (split_off32(g))
+    launch_split_impl<true>(A, B, nbw, S, g, p, stream, pass);
+  else
+    launch_split_impl<false>(A, B, nbw, S, g, p, stream, pass);
 }
 
 }  // namespace dlbc

@@ -383,3 +383,18 @@ def test_laplace_equals_oracle(D, oracle_mod):
     a, b = g.laplace(), o.laplace(p["gcc"], p["psiForm"])
     for k in ("sigma", "radius", "in_pressure", "out_pressure"):
         assert abs(a[k] - b[k]) <= 1e-13 * abs(b[k]), (k, a[k], b[k])
+
+
+def test_split_wide_offsets_equal_fused(D):
+    """A lattice whose population index space exceeds 2^29 (19 x 28.7 M
+    sites) takes the split kernels' 64-bit addressing path; after three steps
+    its fields equal the fused schedule's bit for bit."""
+    n = [232, 352, 352]
+    p = _walls_case(n, "BDist2")
+    a = D.Lattice.from_params(p, schedule="split")
+    b = D.Lattice.from_params(p, schedule="fused")
+    a.step(3)
+    b.step(3)
+    for f in range(2):
+        assert_close(a.density(f), b.density(f), 0.0, "density f%d" % f)
+        assert_close(a.velocity(f), b.velocity(f), 0.0, "velocity f%d" % f)
