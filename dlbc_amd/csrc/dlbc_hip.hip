@@ -446,15 +446,25 @@ struct SplitStep {
   static int run(dlbc_gpu_ctx *c, Timer *tm, double *acc_ms) {
     (void)acc_ms;
     if constexpr (std::is_same<L, D3Q19>::value && C == 2) {
-      const double *A = static_cast<const double *>(c->popA);
-      double *B = static_cast<double *>(c->popB);
+      auto pass = [&](int k) {
+        if (c->storage == DLBC_STORAGE_F64)
+          launch_split_d3q19(static_cast<const double *>(c->popA),
+                             static_cast<double *>(c->popB), c->nbw, c->S, c->geo, c->p,
+                             c->stream, k);
+        else if (c->storage == DLBC_STORAGE_F32_SHIFT)
+          launch_ssplit_d3q19<float>(c->popA, c->popB, c->nbw, c->S, c->geo, c->sparams,
+                                     c->stream, k);
+        else
+          launch_ssplit_d3q19<__half>(c->popA, c->popB, c->nbw, c->S, c->geo, c->sparams,
+                                      c->stream, k);
+      };
       if (c->p.sc) {
         tm->before();
-        launch_split_d3q19(A, B, c->nbw, c->S, c->geo, c->p, c->stream, 0);
+        pass(0);
         tm->after();
       }
       tm->before();
-      launch_split_d3q19(A, B, c->nbw, c->S, c->geo, c->p, c->stream, 1);
+      pass(1);
       tm->after();
       std::swap(c->popA, c->popB);
       return DLBC_OK;
@@ -497,8 +507,8 @@ int with_loader(dlbc_gpu_ctx *c, Fn fn) {
 }
 
 int effective_schedule(const dlbc_gpu_ctx *c) {
-  if (c->storage != DLBC_STORAGE_F64) return DLBC_SCHED_FUSED;
   if (c->schedule != DLBC_SCHED_AUTO) return c->schedule;
+  if (c->storage != DLBC_STORAGE_F64) return DLBC_SCHED_FUSED;
   // D3Q19 two-component FP64: the lane-per-component two-pass kernels are the
   // fastest measured schedule (256^3 channel: 3.45 ms/step vs 4.51 ms fused)
   if (c->d == 3 && c->C == 2) return DLBC_SCHED_SPLIT;
@@ -966,8 +976,13 @@ const char *dlbc_gpu_kernel_name(const dlbc_gpu_ctx *c, int i) {
     return i == 0 ? "tiled_d3q19_shifted" : "";
   if (sched == DLBC_SCHED_FUSED) return i == 0 ? "fused_d3q19" : "";
   if (sched == DLBC_SCHED_SPLIT) {
-    if (c->p.sc) return i == 0 ? "psi_split" : (i == 1 ? "stream_force_collide_split" : "");
-    return i == 0 ? "stream_collide_split" : "";
+    const bool sh = c->storage != DLBC_STORAGE_F64;
+    if (c->p.sc)
+      return i == 0 ? (sh ? "psi_split_shifted" : "psi_split")
+                    : (i == 1 ? (sh ? "stream_force_collide_split_shifted"
+                                    : "stream_force_collide_split")
+                              : "");
+    return i == 0 ? (sh ? "stream_collide_split_shifted" : "stream_collide_split") : "";
   }
   if (c->p.sc) return i == 0 ? "psi" : (i == 1 ? "stream_force_collide" : "");
   return i == 0 ? "stream_collide" : "";
@@ -983,8 +998,6 @@ int dlbc_gpu_set_schedule(dlbc_gpu_ctx *c, int schedule) {
   if (!c || schedule < 0 || schedule > 3) return DLBC_ERR_ARG;
   if (schedule == DLBC_SCHED_SPLIT && !(c->d == 3 && c->C == 2))
     return fail(c, DLBC_ERR_UNSUPPORTED, "split schedule needs D3Q19 with 2 components");
-  if (schedule == DLBC_SCHED_SPLIT && c->storage != DLBC_STORAGE_F64)
-    return fail(c, DLBC_ERR_UNSUPPORTED, "the split schedule needs FP64 storage");
   if (schedule == DLBC_SCHED_FUSED && c->storage == DLBC_STORAGE_F64 &&
       !(c->d == 3 && c->C == 2 && fused_d3q19_supported(c->geo)))
     return fail(c, DLBC_ERR_UNSUPPORTED,

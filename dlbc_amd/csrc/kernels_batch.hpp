@@ -80,13 +80,7 @@ __device__ __forceinline__ void batch_tile(const BatchItem &item, const BatchLat
     // Parameters: uniform ones read from the table, per-component ones selected
     // into registers once per item (a lane holds component f).
     const ShiftParams &p = B.p;
-    auto sel = [f](float a0, float a1) { return f ? a1 : a0; };
-    const float tau_f = sel(p.tau[0], p.tau[1]), tau_o = sel(p.tau[1], p.tau[0]);
-    const float om_f = sel(p.omega[0], p.omega[1]);
-    const float g_self = sel(p.gcc[0][0], p.gcc[1][1]), g_other = sel(p.gcc[0][1], p.gcc[1][0]);
-    const float wc_f = sel(p.gwc[0], p.gwc[1]);
-    const float body_f[3] = {sel(p.body[0][0], p.body[1][0]), sel(p.body[0][1], p.body[1][1]),
-                             sel(p.body[0][2], p.body[1][2])};
+    const ShiftLane pl = shift_lane(p, f);
     const float rho0_0 = p.rho0[0], rho0_1 = p.rho0[1];
     const int psi_form = p.psi_form;
     const Store *__restrict__ A = static_cast<const Store *>(B.pop[parity]);
@@ -104,7 +98,7 @@ __device__ __forceinline__ void batch_tile(const BatchItem &item, const BatchLat
     const int y = y0 + ly, z = z0 + lz;
     const Store *__restrict__ Af = A + (int64_t)f * Q * sp;
     Store *__restrict__ Wf = W + (int64_t)f * Q * sp;
-    const float rho0 = sel(rho0_0, rho0_1);
+    const float rho0 = pl.rho0;
 
     auto gsite = [&](int x, int yy, int zz) { return (x * ny + yy) * nz + zz; };
     // bit 31 set: nothing to do at this lane and plane (solid site or no site)
@@ -129,12 +123,7 @@ __device__ __forceinline__ void batch_tile(const BatchItem &item, const BatchLat
         n[i] = to_f(Af[(int64_t)comp * sp + src]);
       }
     };
-    auto drho = [](const float *n) {
-      float s = 0.0f;
-#pragma unroll
-      for (int i = 0; i < Q; ++i) s += n[i];
-      return s;
-    };
+    auto drho = [](const float *n) { return shifted_drho(n); };
     auto stage_ring = [&](int x, float *stg) {
       for (int w = t; w < ring_items; w += Sh::threads) {
         const int fq = w / ring, r = w - fq * ring;
@@ -218,90 +207,27 @@ __device__ __forceinline__ void batch_tile(const BatchItem &item, const BatchLat
       __syncthreads();
       {
         float j[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) j[d] = 0.0f;
-#pragma unroll
-        for (int i = 0; i < Q; ++i)
-#pragma unroll
-          for (int d = 0; d < 3; ++d)
-            if (L::c[i][d] != 0) j[d] += (float)L::c[i][d] * cur[i];
+        shifted_moment(cur, j);
         const float rho_p = shfl_xor32f(rho_cur);
-        float j_p[3];
-        const bool bdist2 = p.eq_form == kBDist2;
-        if (bdist2) {
+        float j_p[3] = {0.0f, 0.0f, 0.0f};
+        if (p.eq_form == kBDist2) {
 #pragma unroll
           for (int d = 0; d < 3; ++d) j_p[d] = shfl_xor32f(j[d]);
         }
         if (!(sw_cur >> 31)) {
-          float F[3] = {0.0f, 0.0f, 0.0f};
-          const float psi1 = psi_lds[pslot(x)][f][me_e];
-          if (p.sc) {
-            // the other component first, then the own one: for the usual
-            // coupling matrix (zero diagonal) both lanes of a site take the
-            // same branch
-#pragma unroll
-            for (int which = 0; which < 2; ++which) {
-              const int f2 = which ? f : (f ^ 1);
-              const float cc = which ? g_self : g_other;
-              if (cc == 0.0f) continue;
-              const float psi_self = psi_lds[pslot(x)][f2][me_e];
-              float acc[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-              for (int i = 1; i < Q; ++i) {
-                const bool sn = (sw_cur >> i) & 1u;
-                const float psi2 =
-                    sn ? psi_self
-                       : psi_lds[pslot(x + L::c[i][0])][f2][me_e + L::c[i][1] * EZ + L::c[i][2]];
-                const float wp = (float)L::w[i] * psi2;
-#pragma unroll
-                for (int d = 0; d < 3; ++d)
-                  if (L::c[i][d] != 0) acc[d] += (float)L::c[i][d] * wp;
-              }
-#pragma unroll
-              for (int d = 0; d < 3; ++d) F[d] -= psi1 * cc * acc[d];
-            }
-            const float wc = wc_f;
-            if (solid && wc != 0.0f) {
-#pragma unroll
-              for (int i = 0; i < Q; ++i) {
-                if ((sw_cur >> i) & 1u) {
-                  const int nb = gsite(wrapf(x + L::c[i][0], nx), wrapf(y + L::c[i][1], ny),
-                                       wrapf(z + L::c[i][2], nz));
-                  const float pf = (float)L::w[i] * psi1 * (float)B.S[f * sp + nb] * wc;
-#pragma unroll
-                  for (int d = 0; d < 3; ++d) F[d] -= pf * (float)L::c[i][d];
-                }
-              }
-            }
-          }
-          if (p.has_body) {
-#pragma unroll
-            for (int d = 0; d < 3; ++d) F[d] += body_f[d];
-          }
-          const float rinv = 1.0f / rho_cur;
-          float u[3];
-          if (bdist2) {
-            const float tot_inv = 1.0f / (rho_cur + rho_p);
-            const float tau_p = tau_o;
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-              const float wv = (j[d] / tau_f + j_p[d] / tau_p) * tot_inv;
-              u[d] = tau_f * (p.gacc[d] + F[d] * rinv) + wv;
-            }
-          } else {
-#pragma unroll
-            for (int d = 0; d < 3; ++d) u[d] = tau_f * (p.gacc[d] + F[d] * rinv) + j[d] * rinv;
-          }
-          float e[Q];
-          eq_dev_d3q19(p.eq_form, u, e);
-          const float drh = rho_cur - rho0;
-          const float om = om_f;
           const int s = x * plane + y * nz + z;
-#pragma unroll
-          for (int i = 0; i < Q; ++i) {
-            const float deq = (float)L::w[i] * drh + rho_cur * e[i];
-            store_nt<Store>(&Wf[(int64_t)i * sp + s], cur[i] + om * (deq - cur[i]));
-          }
+          shifted_collide_site<SOLID>(
+              f, cur, rho_cur, rho_p, j, j_p, sw_cur, p, pl,
+              [&](int ff, int i) {
+                if (i < 0) return psi_lds[pslot(x)][ff][me_e];
+                return psi_lds[pslot(x + L::c[i][0])][ff][me_e + L::c[i][1] * EZ + L::c[i][2]];
+              },
+              [&](int i) {
+                const int nb = gsite(wrapf(x + L::c[i][0], nx), wrapf(y + L::c[i][1], ny),
+                                     wrapf(z + L::c[i][2], nz));
+                return (float)B.S[f * sp + nb];
+              },
+              [&](int i, float v) { store_nt<Store>(&Wf[(int64_t)i * sp + s], v); });
         }
       }
 #pragma unroll

@@ -121,4 +121,121 @@ __device__ inline void eq_dev_d3q19(int form, const float (&u)[3], float (&e)[19
 
 __device__ inline float shfl_xor32f(float v) { return __shfl_xor(v, 32); }
 
+// The parameters of the component a lane holds, selected once.
+struct ShiftLane {
+  float rho0, tau_f, tau_o, om_f, g_self, g_other, wc_f, body_f[3];
+};
+
+__device__ inline ShiftLane shift_lane(const ShiftParams &p, int f) {
+  auto sel = [f](float a0, float a1) { return f ? a1 : a0; };
+  ShiftLane l;
+  l.rho0 = sel(p.rho0[0], p.rho0[1]);
+  l.tau_f = sel(p.tau[0], p.tau[1]);
+  l.tau_o = sel(p.tau[1], p.tau[0]);
+  l.om_f = sel(p.omega[0], p.omega[1]);
+  l.g_self = sel(p.gcc[0][0], p.gcc[1][1]);
+  l.g_other = sel(p.gcc[0][1], p.gcc[1][0]);
+  l.wc_f = sel(p.gwc[0], p.gwc[1]);
+  for (int d = 0; d < 3; ++d) l.body_f[d] = sel(p.body[0][d], p.body[1][d]);
+  return l;
+}
+
+// Sum of the 19 stored deviations in i order (rho - rho0).
+__device__ inline float shifted_drho(const float *n) {
+  float s = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 19; ++i) s += n[i];
+  return s;
+}
+
+// First moment sum_i d_i c_i over the c_i != 0 terms in i order.
+__device__ inline void shifted_moment(const float *n, float (&j)[3]) {
+  using L = D3Q19;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) j[d] = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 19; ++i)
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      if (L::c[i][d] != 0) j[d] += (float)L::c[i][d] * n[i];
+}
+
+// Shan-Chen and wall force (force.d:238-309), body force, velocity shift,
+// equilibrium and BGK relaxation of component f at one fluid site, in
+// deviation form.  cur: streamed deviations; rho_p, j_p: the partner
+// component's density and first moment; sw: the site's solid word;
+// psi(ff, i): psi_ff at x + c_i (i < 0: at x); wall_rho(i): rho_f at the solid
+// neighbour x + c_i; out(i, d): the new deviation of population i.  Every
+// schedule of shifted storage calls this, so they all give the same bits.
+template <bool SOLID, class Psi, class WallRho, class Out>
+__device__ inline void shifted_collide_site(int f, const float *cur, float rho_cur, float rho_p,
+                                            const float (&j)[3], const float (&j_p)[3],
+                                            uint32_t sw, const ShiftParams &p,
+                                            const ShiftLane &pl, Psi psi, WallRho wall_rho,
+                                            Out out) {
+  using L = D3Q19;
+  constexpr int Q = 19;
+  float F[3] = {0.0f, 0.0f, 0.0f};
+  const float psi1 = psi(f, -1);
+  if (p.sc) {
+    // the other component first, then the own one: for the usual coupling
+    // matrix (zero diagonal) both lanes of a site take the same branch
+#pragma unroll
+    for (int which = 0; which < 2; ++which) {
+      const int f2 = which ? f : (f ^ 1);
+      const float cc = which ? pl.g_self : pl.g_other;
+      if (cc == 0.0f) continue;
+      const float psi_self = psi(f2, -1);
+      float acc[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int i = 1; i < Q; ++i) {
+        const bool sn = SOLID && ((sw >> i) & 1u);
+        const float psi2 = sn ? psi_self : psi(f2, i);
+        const float wp = (float)L::w[i] * psi2;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          if (L::c[i][d] != 0) acc[d] += (float)L::c[i][d] * wp;
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) F[d] -= psi1 * cc * acc[d];
+    }
+    const float wc = pl.wc_f;
+    if (SOLID && wc != 0.0f) {
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        if ((sw >> i) & 1u) {
+          const float pf = (float)L::w[i] * psi1 * wall_rho(i) * wc;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) F[d] -= pf * (float)L::c[i][d];
+        }
+      }
+    }
+  }
+  if (p.has_body) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) F[d] += pl.body_f[d];
+  }
+  const float rinv = 1.0f / rho_cur;
+  float u[3];
+  if (p.eq_form == kBDist2) {
+    const float tot_inv = 1.0f / (rho_cur + rho_p);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const float wv = (j[d] / pl.tau_f + j_p[d] / pl.tau_o) * tot_inv;
+      u[d] = pl.tau_f * (p.gacc[d] + F[d] * rinv) + wv;
+    }
+  } else {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) u[d] = pl.tau_f * (p.gacc[d] + F[d] * rinv) + j[d] * rinv;
+  }
+  float e[Q];
+  eq_dev_d3q19(p.eq_form, u, e);
+  const float drh = rho_cur - pl.rho0;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const float deq = (float)L::w[i] * drh + rho_cur * e[i];
+    out(i, cur[i] + pl.om_f * (deq - cur[i]));
+  }
+}
+
 }  // namespace dlbc

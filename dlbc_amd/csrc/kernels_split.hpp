@@ -21,6 +21,7 @@
 #pragma once
 
 #include "kernels_fused.hpp"
+#include "kernels_shifted.hpp"
 
 namespace dlbc {
 
@@ -275,6 +276,109 @@ __global__ void __launch_bounds__(kSplitThreads)
   collide_d3q19_streamed(f, n, rho, total, F, wv, p, [&](int i, double v) {
     __builtin_nontemporal_store(v, elem<OFF32>(Bf, (int64_t)i * g.sp + s));
   });
+}
+
+// ---- shifted storage (FP32 / FP16 deviations, FP32 arithmetic) -------------
+// The same two passes; psi goes through S (FP64 array, exact for FP32 values)
+// and the per-site arithmetic is shifted_collide_site, as in the tiled kernel,
+// so a lattice gives the same bits under both schedules.
+
+template <class Store, bool SOLID>
+__device__ inline void ssplit_pull(const Store *__restrict__ Af, const Geo &g, int s, int x,
+                                   int y, int z, uint32_t sw, float *n) {
+  using L = D3Q19;
+#pragma unroll
+  for (int i = 0; i < L::q; ++i) {
+    const bool bb = SOLID && ((sw >> L::bounce[i]) & 1u);
+    const int src = bb ? s
+                       : site_of(wrap(x - L::c[i][0], g.nx), wrap(y - L::c[i][1], g.ny),
+                                 wrap(z - L::c[i][2], g.nz), g);
+    const int comp = bb ? L::bounce[i] : i;
+    n[i] = to_f(Af[(int64_t)comp * g.sp + src]);
+  }
+}
+
+template <class Store, bool SOLID>
+__global__ void __launch_bounds__(kSplitThreads)
+    k_ssplit_psi(const Store *__restrict__ A, const uint32_t *__restrict__ nbw,
+                 double *__restrict__ S, Geo g, ShiftParams p) {
+  int s, f;
+  if (!split_lane(g, s, f)) return;
+  const uint32_t sw = SOLID ? nbw[s] : 0u;
+  if (sw >> 31) return;  // solid: S keeps the site's density
+  int x, y, z;
+  coords(s, g, x, y, z);
+  float n[19];
+  ssplit_pull<Store, SOLID>(A + (int64_t)f * 19 * g.sp, g, s, x, y, z, sw, n);
+  const float rho0 = f ? p.rho0[1] : p.rho0[0];
+  S[f * g.sp + s] = (double)psi_f(p.psi_form, rho0 + shifted_drho(n));
+}
+
+template <class Store, bool SOLID>
+__global__ void __launch_bounds__(kSplitThreads)
+    k_ssplit_collide(const Store *__restrict__ A, Store *__restrict__ B,
+                     const uint32_t *__restrict__ nbw, const double *__restrict__ S, Geo g,
+                     ShiftParams p) {
+  using L = D3Q19;
+  __shared__ float xch[2][4][kSplitThreads / 2];
+  int s, f;
+  const bool inside = split_lane(g, s, f);
+  const int k = ((threadIdx.x >> 7) << 6) | (threadIdx.x & 63);
+  const uint32_t sw = !inside ? (1u << 31) : (SOLID ? nbw[s] : 0u);
+  const ShiftLane pl = shift_lane(p, f);
+  int x = 0, y = 0, z = 0;
+  float n[19];
+  float rho = 0.0f, j[3] = {0.0f, 0.0f, 0.0f};
+  if (!(sw >> 31)) {
+    coords(s, g, x, y, z);
+    ssplit_pull<Store, SOLID>(A + (int64_t)f * 19 * g.sp, g, s, x, y, z, sw, n);
+    rho = pl.rho0 + shifted_drho(n);
+    shifted_moment(n, j);
+  }
+  xch[f][0][k] = rho;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) xch[f][1 + d][k] = j[d];
+  __syncthreads();
+  if (sw >> 31) return;  // solid populations stay as they are
+  const float rho_p = xch[f ^ 1][0][k];
+  float j_p[3] = {0.0f, 0.0f, 0.0f};
+  if (p.eq_form == kBDist2) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) j_p[d] = xch[f ^ 1][1 + d][k];
+  }
+  auto nbr = [&](int i) {
+    return site_of(wrap(x + L::c[i][0], g.nx), wrap(y + L::c[i][1], g.ny),
+                   wrap(z + L::c[i][2], g.nz), g);
+  };
+  Store *__restrict__ Bf = B + (int64_t)f * 19 * g.sp;
+  shifted_collide_site<SOLID>(
+      f, n, rho, rho_p, j, j_p, sw, p, pl,
+      [&](int ff, int i) { return (float)S[ff * g.sp + (i < 0 ? s : nbr(i))]; },
+      [&](int i) { return (float)S[f * g.sp + nbr(i)]; },
+      [&](int i, float v) { store_nt<Store>(&Bf[(int64_t)i * g.sp + s], v); });
+}
+
+template <class Store>
+inline void launch_ssplit_d3q19(const void *A, void *B, const uint32_t *nbw, double *S,
+                                const Geo &g, const ShiftParams &p, hipStream_t stream,
+                                int pass) {
+  const int nb = (g.sites + kSplitThreads / 2 - 1) / (kSplitThreads / 2);
+  const dim3 grid(nb), block(kSplitThreads);
+  const Store *a = static_cast<const Store *>(A);
+  Store *b = static_cast<Store *>(B);
+  if (pass == 0) {
+    if (p.has_solid)
+      hipLaunchKernelGGL((k_ssplit_psi<Store, true>), grid, block, 0, stream, a, nbw, S, g, p);
+    else
+      hipLaunchKernelGGL((k_ssplit_psi<Store, false>), grid, block, 0, stream, a, nbw, S, g, p);
+  } else {
+    if (p.has_solid)
+      hipLaunchKernelGGL((k_ssplit_collide<Store, true>), grid, block, 0, stream, a, b, nbw, S,
+                         g, p);
+    else
+      hipLaunchKernelGGL((k_ssplit_collide<Store, false>), grid, block, 0, stream, a, b, nbw, S,
+                         g, p);
+  }
 }
 
 template <bool OFF32>
