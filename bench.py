@@ -303,7 +303,7 @@ def ensemble_workload(D, count, steps, storage="f32_shift"):
     finite = all(np.isfinite(L.total_mass(0)) for L in lats)
     res = {"lattices": count, "sites": sites, "steps": steps, "storage": storage,
            "config": {"tile_rows": cfg[0], "chunk": cfg[1], "workgroups": cfg[2],
-                      "kernel": ["solid-capable", "no-solid"][cfg[3]]},
+                      "kernel": ["tiled, solid-capable", "tiled, no-solid", "split"][cfg[3]]},
            "autotune_launch_ms": round(ms, 4),
            "batched_graph_mlups": round(sites * steps / dt_graph / 1e6, 1),
            "batched_eager_mlups": round(sites * steps / dt_eager / 1e6, 1),

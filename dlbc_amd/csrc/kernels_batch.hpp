@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "kernels_shifted.hpp"
+#include "kernels_split.hpp"
 
 namespace dlbc {
 
@@ -312,6 +313,65 @@ inline int launch_batch_d3q19(const BatchLattice *lats, const BatchItem *items, 
   else
     return -1;
   return 0;
+}
+
+// ---- batches on the split schedule -------------------------------------------
+// Two launches per step over all members: pass 1 (psi) then pass 2 (collide).
+// Block b of a launch belongs to the lattice l with first[l] <= blk0 + b <
+// first[l + 1] (first: the prefix sums of the members' 128-site block counts).
+
+__device__ inline int batch_block_lattice(const int *__restrict__ first, int nlat, int b) {
+  int lo = 0, hi = nlat;  // first[lo] <= b < first[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (first[mid] <= b)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+template <class Store>
+__global__ void __launch_bounds__(kSplitThreads)
+    k_batch_split_psi(const BatchLattice *__restrict__ lats, const int *__restrict__ first,
+                      int nlat, int blk0, int parity) {
+  const int b = blk0 + blockIdx.x;
+  const int l = batch_block_lattice(first, nlat, b);
+  const BatchLattice &B = lats[l];
+  const Store *A = static_cast<const Store *>(B.pop[parity]);
+  double *S = const_cast<double *>(B.S);
+  if (B.p.has_solid)
+    ssplit_psi_block<Store, true>(A, B.nbw, S, B.g, B.p, b - first[l]);
+  else
+    ssplit_psi_block<Store, false>(A, B.nbw, S, B.g, B.p, b - first[l]);
+}
+
+template <class Store>
+__global__ void __launch_bounds__(kSplitThreads)
+    k_batch_split_collide(const BatchLattice *__restrict__ lats, const int *__restrict__ first,
+                          int nlat, int blk0, int parity) {
+  __shared__ float xch[2][4][kSplitThreads / 2];
+  const int b = blk0 + blockIdx.x;
+  const int l = batch_block_lattice(first, nlat, b);
+  const BatchLattice &B = lats[l];
+  const Store *A = static_cast<const Store *>(B.pop[parity]);
+  Store *W = static_cast<Store *>(const_cast<void *>(B.pop[parity ^ 1]));
+  if (B.p.has_solid)
+    ssplit_collide_block<Store, true>(A, W, B.nbw, B.S, B.g, B.p, b - first[l], xch);
+  else
+    ssplit_collide_block<Store, false>(A, W, B.nbw, B.S, B.g, B.p, b - first[l], xch);
+}
+
+// One step of the blocks [blk0, blk0 + nblk) of a batch on the split schedule.
+template <class Store>
+inline void launch_batch_split(const BatchLattice *lats, const int *first, int nlat, int blk0,
+                               int nblk, int parity, hipStream_t stream) {
+  if (nblk <= 0) return;
+  hipLaunchKernelGGL((k_batch_split_psi<Store>), dim3(nblk), dim3(kSplitThreads), 0, stream,
+                     lats, first, nlat, blk0, parity);
+  hipLaunchKernelGGL((k_batch_split_collide<Store>), dim3(nblk), dim3(kSplitThreads), 0, stream,
+                     lats, first, nlat, blk0, parity);
 }
 
 }  // namespace dlbc

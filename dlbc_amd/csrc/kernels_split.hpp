@@ -298,12 +298,16 @@ __device__ inline void ssplit_pull(const Store *__restrict__ Af, const Geo &g, i
   }
 }
 
+// Pass 1 for block blk of a lattice (the sites blk * 128 ..).
 template <class Store, bool SOLID>
-__global__ void __launch_bounds__(kSplitThreads)
-    k_ssplit_psi(const Store *__restrict__ A, const uint32_t *__restrict__ nbw,
-                 double *__restrict__ S, Geo g, ShiftParams p) {
-  int s, f;
-  if (!split_lane(g, s, f)) return;
+__device__ inline void ssplit_psi_block(const Store *__restrict__ A,
+                                        const uint32_t *__restrict__ nbw,
+                                        double *__restrict__ S, const Geo &g,
+                                        const ShiftParams &p, int blk) {
+  const int w = threadIdx.x >> 6;
+  const int f = w & 1;
+  const int s = blk * (kSplitThreads / 2) + (w >> 1) * 64 + (threadIdx.x & 63);
+  if (s >= g.sites) return;
   const uint32_t sw = SOLID ? nbw[s] : 0u;
   if (sw >> 31) return;  // solid: S keeps the site's density
   int x, y, z;
@@ -314,17 +318,19 @@ __global__ void __launch_bounds__(kSplitThreads)
   S[f * g.sp + s] = (double)psi_f(p.psi_form, rho0 + shifted_drho(n));
 }
 
+// Pass 2 for block blk of a lattice; xch: LDS for the partner exchange.
 template <class Store, bool SOLID>
-__global__ void __launch_bounds__(kSplitThreads)
-    k_ssplit_collide(const Store *__restrict__ A, Store *__restrict__ B,
-                     const uint32_t *__restrict__ nbw, const double *__restrict__ S, Geo g,
-                     ShiftParams p) {
+__device__ inline void ssplit_collide_block(const Store *__restrict__ A, Store *__restrict__ B,
+                                            const uint32_t *__restrict__ nbw,
+                                            const double *__restrict__ S, const Geo &g,
+                                            const ShiftParams &p, int blk,
+                                            float (*xch)[4][kSplitThreads / 2]) {
   using L = D3Q19;
-  __shared__ float xch[2][4][kSplitThreads / 2];
-  int s, f;
-  const bool inside = split_lane(g, s, f);
+  const int w = threadIdx.x >> 6;
+  const int f = w & 1;
+  const int s = blk * (kSplitThreads / 2) + (w >> 1) * 64 + (threadIdx.x & 63);
   const int k = ((threadIdx.x >> 7) << 6) | (threadIdx.x & 63);
-  const uint32_t sw = !inside ? (1u << 31) : (SOLID ? nbw[s] : 0u);
+  const uint32_t sw = s >= g.sites ? (1u << 31) : (SOLID ? nbw[s] : 0u);
   const ShiftLane pl = shift_lane(p, f);
   int x = 0, y = 0, z = 0;
   float n[19];
@@ -356,6 +362,22 @@ __global__ void __launch_bounds__(kSplitThreads)
       [&](int ff, int i) { return (float)S[ff * g.sp + (i < 0 ? s : nbr(i))]; },
       [&](int i) { return (float)S[f * g.sp + nbr(i)]; },
       [&](int i, float v) { store_nt<Store>(&Bf[(int64_t)i * g.sp + s], v); });
+}
+
+template <class Store, bool SOLID>
+__global__ void __launch_bounds__(kSplitThreads)
+    k_ssplit_psi(const Store *__restrict__ A, const uint32_t *__restrict__ nbw,
+                 double *__restrict__ S, Geo g, ShiftParams p) {
+  ssplit_psi_block<Store, SOLID>(A, nbw, S, g, p, blockIdx.x);
+}
+
+template <class Store, bool SOLID>
+__global__ void __launch_bounds__(kSplitThreads)
+    k_ssplit_collide(const Store *__restrict__ A, Store *__restrict__ B,
+                     const uint32_t *__restrict__ nbw, const double *__restrict__ S, Geo g,
+                     ShiftParams p) {
+  __shared__ float xch[2][4][kSplitThreads / 2];
+  ssplit_collide_block<Store, SOLID>(A, B, nbw, S, g, p, blockIdx.x, xch);
 }
 
 template <class Store>

@@ -197,9 +197,11 @@ typedef struct dlbc_batch dlbc_batch;
 
 int dlbc_batch_create(dlbc_batch **out, dlbc_gpu_ctx *const *lattices, int n);
 /* Tile height (4 or 8 rows of 32 sites), x-chunk length (planes), number of
- * workgroups of the persistent launch and kernel (0: solid-capable, 128
- * VGPRs; 1: for batches without solid sites, more loads in flight).
- * Defaults 8, whole lattice, 512, 0. */
+ * workgroups of the persistent launch and kernel: 0 the tiled kernel
+ * (solid-capable, 128 VGPRs), 1 the tiled kernel for batches without solid
+ * sites (more loads in flight), 2 the split pair (a psi launch and a
+ * stream/force/collide launch per step over all members; tile, chunk and
+ * workgroups do not apply).  Defaults 8, whole lattice, 512, 0. */
 int dlbc_batch_set_config(dlbc_batch *b, int tile_y, int chunk, int workgroups, int kernel);
 int dlbc_batch_get_config(const dlbc_batch *b, int *out /*4*/);
 /* Time every (tile height, chunk, workgroups, kernel) candidate on the

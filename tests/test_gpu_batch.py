@@ -48,8 +48,9 @@ def _same(a, b, what):
             assert np.array_equal(pa[f].view(np.uint64), pb[f].view(np.uint64)), (what, k, f)
 
 
+@pytest.mark.parametrize("kernel", [0, 2], ids=["tiled", "split"])
 @pytest.mark.parametrize("storage", ["f32_shift", "f16_shift"])
-def test_batch_equals_lattices_stepped_alone(D, storage):
+def test_batch_equals_lattices_stepped_alone(D, storage, kernel):
     """11 steps (odd: the buffers end swapped) in one batch, graph replay on,
     equal bit for bit to stepping every lattice alone; the per-lattice loop
     gives the same bits too."""
@@ -59,19 +60,25 @@ def test_batch_equals_lattices_stepped_alone(D, storage):
     ref = _pops(alone)
     lats = _members(D, storage)
     B = D.Batch(lats)
+    B.set_config(8, 1 << 20, 512, kernel)
     B.step(11, graph=True)
     _same(_pops(lats), ref, "batch")
     lats2 = _members(D, storage)
-    D.Batch(lats2).step_looped(11)
+    B2 = D.Batch(lats2)
+    B2.set_config(8, 1 << 20, 512, kernel)
+    B2.step_looped(11)
     _same(_pops(lats2), ref, "looped")
 
 
-def test_batch_graph_replays(D):
+@pytest.mark.parametrize("kernel", [0, 2], ids=["tiled", "split"])
+def test_batch_graph_replays(D, kernel):
     """The captured two-step graph replayed several times, with eager work
     (field downloads, an eager step, set_params) between the replays: every
     replay matches the same sequence run eagerly."""
     a, b = _members(D), _members(D)
     Ba, Bb = D.Batch(a), D.Batch(b)
+    Ba.set_config(8, 1 << 20, 512, kernel)
+    Bb.set_config(8, 1 << 20, 512, kernel)
     for k in range(4):
         Ba.step(4, graph=True)
         Bb.step(4, graph=False)
@@ -92,7 +99,8 @@ def test_batch_graph_replays(D):
 
 def test_batch_config_and_autotune_do_not_change_results(D):
     ref = None
-    for cfg in [(8, 1 << 20, 512, 0), (4, 16, 64, 1), (8, 8, 2048, 1), (4, 32, 8, 0)]:
+    for cfg in [(8, 1 << 20, 512, 0), (4, 16, 64, 1), (8, 8, 2048, 1), (4, 32, 8, 0),
+                (8, 16, 256, 2)]:
         lats = _members(D)
         B = D.Batch(lats)
         B.set_config(*cfg)
