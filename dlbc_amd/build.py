@@ -57,5 +57,40 @@ def build(verbose=False):
     return LIB
 
 
+H5_LIB = os.path.join(HERE, "libdlbc_h5.so")
+H5_PREFIXES = [os.environ.get("DLBC_HDF5_PREFIX"), "/opt/conda", "/usr"]
+
+
+def hdf5_prefix():
+    for p in H5_PREFIXES:
+        if p and os.path.exists(os.path.join(p, "include", "hdf5.h")):
+            for lib in ("lib", "lib64", "lib/x86_64-linux-gnu"):
+                if os.path.exists(os.path.join(p, lib, "libhdf5.so")):
+                    return p, os.path.join(p, lib)
+    return None, None
+
+
+def build_h5(verbose=False):
+    """libdlbc_h5.so (the reference's HDF5 file layout) with the host C++
+    compiler against the HDF5 C library; None when no HDF5 is installed."""
+    prefix, libdir = hdf5_prefix()
+    if prefix is None:
+        if verbose:
+            print("HDF5 not found: libdlbc_h5.so not built", file=sys.stderr)
+        return None
+    tmp = H5_LIB + ".tmp"
+    cmd = [shutil.which("g++") or "g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall",
+           os.path.join(CSRC, "h5io.cpp"), "-I", os.path.join(prefix, "include"),
+           "-L", libdir, "-Wl,-rpath," + libdir, "-lhdf5", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build of libdlbc_h5.so failed:\n%s" % r.stderr)
+    os.replace(tmp, H5_LIB)
+    return H5_LIB
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_h5(verbose=True))
