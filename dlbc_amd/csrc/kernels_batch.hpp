@@ -342,9 +342,9 @@ __global__ void __launch_bounds__(kSplitThreads)
   const Store *A = static_cast<const Store *>(B.pop[parity]);
   double *S = const_cast<double *>(B.S);
   if (B.p.has_solid)
-    ssplit_psi_block<Store, true>(A, B.nbw, S, B.g, B.p, b - first[l]);
+    ssplit_psi_block<Store, true, false>(A, B.nbw, S, B.g, B.p, b - first[l]);
   else
-    ssplit_psi_block<Store, false>(A, B.nbw, S, B.g, B.p, b - first[l]);
+    ssplit_psi_block<Store, false, false>(A, B.nbw, S, B.g, B.p, b - first[l]);
 }
 
 template <class Store>
@@ -358,9 +358,9 @@ __global__ void __launch_bounds__(kSplitThreads)
   const Store *A = static_cast<const Store *>(B.pop[parity]);
   Store *W = static_cast<Store *>(const_cast<void *>(B.pop[parity ^ 1]));
   if (B.p.has_solid)
-    ssplit_collide_block<Store, true>(A, W, B.nbw, B.S, B.g, B.p, b - first[l], xch);
+    ssplit_collide_block<Store, true, false>(A, W, B.nbw, B.S, B.g, B.p, b - first[l], xch);
   else
-    ssplit_collide_block<Store, false>(A, W, B.nbw, B.S, B.g, B.p, b - first[l], xch);
+    ssplit_collide_block<Store, false, false>(A, W, B.nbw, B.S, B.g, B.p, b - first[l], xch);
 }
 
 // One step of the blocks [blk0, blk0 + nblk) of a batch on the split schedule.

@@ -39,16 +39,16 @@ __device__ inline bool split_lane(const Geo &g, int &s, int &f) {
 // Element k of the array at `base` (wave-uniform).  OFF32 (every index of
 // the kernel below 2^29): the byte offset fits 32 bits, so the access takes a
 // scalar base and a 32-bit lane offset instead of a 64-bit lane address.
-template <bool OFF32>
-__device__ inline double *elem(double *base, int64_t k) {
+template <bool OFF32, class T>
+__device__ inline T *elem(T *base, int64_t k) {
   if constexpr (OFF32)
-    return reinterpret_cast<double *>(reinterpret_cast<char *>(base) + ((uint32_t)k << 3));
+    return reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (uint32_t)k * sizeof(T));
   else
     return base + k;
 }
-template <bool OFF32>
-__device__ inline const double &at(const double *base, int64_t k) {
-  return *elem<OFF32>(const_cast<double *>(base), k);
+template <bool OFF32, class T>
+__device__ inline const T &at(const T *base, int64_t k) {
+  return *elem<OFF32>(const_cast<T *>(base), k);
 }
 
 // True when every population index of a lattice (19 sp) is below 2^29.
@@ -283,7 +283,7 @@ __global__ void __launch_bounds__(kSplitThreads)
 // and the per-site arithmetic is shifted_collide_site, as in the tiled kernel,
 // so a lattice gives the same bits under both schedules.
 
-template <class Store, bool SOLID>
+template <class Store, bool SOLID, bool OFF32>
 __device__ inline void ssplit_pull(const Store *__restrict__ Af, const Geo &g, int s, int x,
                                    int y, int z, uint32_t sw, float *n) {
   using L = D3Q19;
@@ -294,12 +294,12 @@ __device__ inline void ssplit_pull(const Store *__restrict__ Af, const Geo &g, i
                        : site_of(wrap(x - L::c[i][0], g.nx), wrap(y - L::c[i][1], g.ny),
                                  wrap(z - L::c[i][2], g.nz), g);
     const int comp = bb ? L::bounce[i] : i;
-    n[i] = to_f(Af[(int64_t)comp * g.sp + src]);
+    n[i] = to_f(at<OFF32>(Af, (int64_t)comp * g.sp + src));
   }
 }
 
 // Pass 1 for block blk of a lattice (the sites blk * 128 ..).
-template <class Store, bool SOLID>
+template <class Store, bool SOLID, bool OFF32>
 __device__ inline void ssplit_psi_block(const Store *__restrict__ A,
                                         const uint32_t *__restrict__ nbw,
                                         double *__restrict__ S, const Geo &g,
@@ -313,13 +313,13 @@ __device__ inline void ssplit_psi_block(const Store *__restrict__ A,
   int x, y, z;
   coords(s, g, x, y, z);
   float n[19];
-  ssplit_pull<Store, SOLID>(A + (int64_t)f * 19 * g.sp, g, s, x, y, z, sw, n);
+  ssplit_pull<Store, SOLID, OFF32>(A + (int64_t)f * 19 * g.sp, g, s, x, y, z, sw, n);
   const float rho0 = f ? p.rho0[1] : p.rho0[0];
-  S[f * g.sp + s] = (double)psi_f(p.psi_form, rho0 + shifted_drho(n));
+  *elem<OFF32>(S, (int64_t)f * g.sp + s) = (double)psi_f(p.psi_form, rho0 + shifted_drho(n));
 }
 
 // Pass 2 for block blk of a lattice; xch: LDS for the partner exchange.
-template <class Store, bool SOLID>
+template <class Store, bool SOLID, bool OFF32>
 __device__ inline void ssplit_collide_block(const Store *__restrict__ A, Store *__restrict__ B,
                                             const uint32_t *__restrict__ nbw,
                                             const double *__restrict__ S, const Geo &g,
@@ -337,7 +337,7 @@ __device__ inline void ssplit_collide_block(const Store *__restrict__ A, Store *
   float rho = 0.0f, j[3] = {0.0f, 0.0f, 0.0f};
   if (!(sw >> 31)) {
     coords(s, g, x, y, z);
-    ssplit_pull<Store, SOLID>(A + (int64_t)f * 19 * g.sp, g, s, x, y, z, sw, n);
+    ssplit_pull<Store, SOLID, OFF32>(A + (int64_t)f * 19 * g.sp, g, s, x, y, z, sw, n);
     rho = pl.rho0 + shifted_drho(n);
     shifted_moment(n, j);
   }
@@ -359,48 +359,60 @@ __device__ inline void ssplit_collide_block(const Store *__restrict__ A, Store *
   Store *__restrict__ Bf = B + (int64_t)f * 19 * g.sp;
   shifted_collide_site<SOLID>(
       f, n, rho, rho_p, j, j_p, sw, p, pl,
-      [&](int ff, int i) { return (float)S[ff * g.sp + (i < 0 ? s : nbr(i))]; },
-      [&](int i) { return (float)S[f * g.sp + nbr(i)]; },
-      [&](int i, float v) { store_nt<Store>(&Bf[(int64_t)i * g.sp + s], v); });
+      [&](int ff, int i) { return (float)at<OFF32>(S, ff * g.sp + (i < 0 ? s : nbr(i))); },
+      [&](int i) { return (float)at<OFF32>(S, f * g.sp + nbr(i)); },
+      [&](int i, float v) { store_nt<Store>(elem<OFF32>(Bf, (int64_t)i * g.sp + s), v); });
 }
 
-template <class Store, bool SOLID>
+template <class Store, bool SOLID, bool OFF32>
 __global__ void __launch_bounds__(kSplitThreads)
     k_ssplit_psi(const Store *__restrict__ A, const uint32_t *__restrict__ nbw,
                  double *__restrict__ S, Geo g, ShiftParams p) {
-  ssplit_psi_block<Store, SOLID>(A, nbw, S, g, p, blockIdx.x);
+  ssplit_psi_block<Store, SOLID, OFF32>(A, nbw, S, g, p, blockIdx.x);
 }
 
-template <class Store, bool SOLID>
+template <class Store, bool SOLID, bool OFF32>
 __global__ void __launch_bounds__(kSplitThreads)
     k_ssplit_collide(const Store *__restrict__ A, Store *__restrict__ B,
                      const uint32_t *__restrict__ nbw, const double *__restrict__ S, Geo g,
                      ShiftParams p) {
   __shared__ float xch[2][4][kSplitThreads / 2];
-  ssplit_collide_block<Store, SOLID>(A, B, nbw, S, g, p, blockIdx.x, xch);
+  ssplit_collide_block<Store, SOLID, OFF32>(A, B, nbw, S, g, p, blockIdx.x, xch);
+}
+
+template <class Store, bool OFF32>
+inline void launch_ssplit_impl(const Store *a, Store *b, const uint32_t *nbw, double *S,
+                               const Geo &g, const ShiftParams &p, hipStream_t stream,
+                               int pass) {
+  const int nb = (g.sites + kSplitThreads / 2 - 1) / (kSplitThreads / 2);
+  const dim3 grid(nb), block(kSplitThreads);
+  if (pass == 0) {
+    if (p.has_solid)
+      hipLaunchKernelGGL((k_ssplit_psi<Store, true, OFF32>), grid, block, 0, stream, a, nbw, S,
+                         g, p);
+    else
+      hipLaunchKernelGGL((k_ssplit_psi<Store, false, OFF32>), grid, block, 0, stream, a, nbw, S,
+                         g, p);
+  } else {
+    if (p.has_solid)
+      hipLaunchKernelGGL((k_ssplit_collide<Store, true, OFF32>), grid, block, 0, stream, a, b,
+                         nbw, S, g, p);
+    else
+      hipLaunchKernelGGL((k_ssplit_collide<Store, false, OFF32>), grid, block, 0, stream, a, b,
+                         nbw, S, g, p);
+  }
 }
 
 template <class Store>
 inline void launch_ssplit_d3q19(const void *A, void *B, const uint32_t *nbw, double *S,
                                 const Geo &g, const ShiftParams &p, hipStream_t stream,
                                 int pass) {
-  const int nb = (g.sites + kSplitThreads / 2 - 1) / (kSplitThreads / 2);
-  const dim3 grid(nb), block(kSplitThreads);
   const Store *a = static_cast<const Store *>(A);
   Store *b = static_cast<Store *>(B);
-  if (pass == 0) {
-    if (p.has_solid)
-      hipLaunchKernelGGL((k_ssplit_psi<Store, true>), grid, block, 0, stream, a, nbw, S, g, p);
-    else
-      hipLaunchKernelGGL((k_ssplit_psi<Store, false>), grid, block, 0, stream, a, nbw, S, g, p);
-  } else {
-    if (p.has_solid)
-      hipLaunchKernelGGL((k_ssplit_collide<Store, true>), grid, block, 0, stream, a, b, nbw, S,
-                         g, p);
-    else
-      hipLaunchKernelGGL((k_ssplit_collide<Store, false>), grid, block, 0, stream, a, b, nbw, S,
-                         g, p);
-  }
+  if (split_off32(g))
+    launch_ssplit_impl<Store, true>(a, b, nbw, S, g, p, stream, pass);
+  else
+    launch_ssplit_impl<Store, false>(a, b, nbw, S, g, p, stream, pass);
 }
 
 template <bool OFF32>
