@@ -335,8 +335,9 @@ __device__ inline int batch_block_lattice(const int *__restrict__ first, int nla
 template <class Store>
 __global__ void __launch_bounds__(kSplitThreads)
     k_batch_split_psi(const BatchLattice *__restrict__ lats, const int *__restrict__ first,
-                      int nlat, int blk0, int parity) {
+                      int nlat, int blk0, int blk1, int parity) {
   const int b = blk0 + blockIdx.x;
+  if (b >= blk1) return;
   const int l = batch_block_lattice(first, nlat, b);
   const BatchLattice &B = lats[l];
   const Store *A = static_cast<const Store *>(B.pop[parity]);
@@ -350,9 +351,10 @@ __global__ void __launch_bounds__(kSplitThreads)
 template <class Store>
 __global__ void __launch_bounds__(kSplitThreads)
     k_batch_split_collide(const BatchLattice *__restrict__ lats, const int *__restrict__ first,
-                          int nlat, int blk0, int parity) {
+                          int nlat, int blk0, int blk1, int parity) {
   __shared__ float xch[2][4][kSplitThreads / 2];
   const int b = blk0 + blockIdx.x;
+  if (b >= blk1) return;  // whole block: no barrier is left waiting
   const int l = batch_block_lattice(first, nlat, b);
   const BatchLattice &B = lats[l];
   const Store *A = static_cast<const Store *>(B.pop[parity]);
@@ -368,10 +370,11 @@ template <class Store>
 inline void launch_batch_split(const BatchLattice *lats, const int *first, int nlat, int blk0,
                                int nblk, int parity, hipStream_t stream) {
   if (nblk <= 0) return;
-  hipLaunchKernelGGL((k_batch_split_psi<Store>), dim3(nblk), dim3(kSplitThreads), 0, stream,
-                     lats, first, nlat, blk0, parity);
-  hipLaunchKernelGGL((k_batch_split_collide<Store>), dim3(nblk), dim3(kSplitThreads), 0, stream,
-                     lats, first, nlat, blk0, parity);
+  const int grid = nblk;
+  hipLaunchKernelGGL((k_batch_split_psi<Store>), dim3(grid), dim3(kSplitThreads), 0, stream,
+                     lats, first, nlat, blk0, blk0 + nblk, parity);
+  hipLaunchKernelGGL((k_batch_split_collide<Store>), dim3(grid), dim3(kSplitThreads), 0, stream,
+                     lats, first, nlat, blk0, blk0 + nblk, parity);
 }
 
 }  // namespace dlbc

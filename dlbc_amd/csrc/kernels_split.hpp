@@ -27,6 +27,13 @@ namespace dlbc {
 
 constexpr int kSplitThreads = 256;  // 4 waves = 128 sites x 2 components
 
+// Grid of a split launch over `sites`: 128-site blocks, in site order.  (A
+// grid reordered so that each XCD walks one contiguous run of sites measured
+// slower: 3.76 against 3.43 ms per FP64 step at 256^3.)
+inline int split_grid(int64_t sites) {
+  return (int)((sites + kSplitThreads / 2 - 1) / (kSplitThreads / 2));
+}
+
 // Site and component of this lane (the component is uniform per wave); false
 // past the end of the lattice.
 __device__ inline bool split_lane(const Geo &g, int &s, int &f) {
@@ -384,7 +391,7 @@ template <class Store, bool OFF32>
 inline void launch_ssplit_impl(const Store *a, Store *b, const uint32_t *nbw, double *S,
                                const Geo &g, const ShiftParams &p, hipStream_t stream,
                                int pass) {
-  const int nb = (g.sites + kSplitThreads / 2 - 1) / (kSplitThreads / 2);
+  const int nb = split_grid(g.sites);
   const dim3 grid(nb), block(kSplitThreads);
   if (pass == 0) {
     if (p.has_solid)
@@ -418,7 +425,7 @@ inline void launch_ssplit_d3q19(const void *A, void *B, const uint32_t *nbw, dou
 template <bool OFF32>
 inline void launch_split_impl(const double *A, double *B, const uint32_t *nbw, double *S,
                               const Geo &g, const StepParams &p, hipStream_t stream, int pass) {
-  const int nb = (g.sites + kSplitThreads / 2 - 1) / (kSplitThreads / 2);
+  const int nb = split_grid(g.sites);
   const dim3 grid(nb), block(kSplitThreads);
   if (pass == 0) {
     if (p.has_solid)
