@@ -36,6 +36,7 @@ struct dlbc_gpu_ctx {
   double rho0[kMaxC] = {1.0, 1.0, 1.0, 1.0};
   ShiftParams sparams{};
   double *S = nullptr;                      // psi (fluid) / rho (solid)
+  float *P = nullptr;                       // shifted storage: psi in FP32 (split schedule)
   uint8_t *mask = nullptr;
   uint32_t *nbw = nullptr;  // solid-neighbour words (bit i: x + c_i solid)
   int has_solid = 0;
@@ -377,6 +378,7 @@ int launch_shifted_self(dlbc_gpu_ctx *c) {
   t.pop[1] = c->bufY;
   t.nbw = c->nbw;
   t.S = c->S;
+  t.P = c->P;
   t.g = c->geo;
   t.p = c->sparams;
   if (!c->self_valid || std::memcmp(&t, &c->self_tab, sizeof t)) {
@@ -452,10 +454,10 @@ struct SplitStep {
                              static_cast<double *>(c->popB), c->nbw, c->S, c->geo, c->p,
                              c->stream, k);
         else if (c->storage == DLBC_STORAGE_F32_SHIFT)
-          launch_ssplit_d3q19<float>(c->popA, c->popB, c->nbw, c->S, c->geo, c->sparams,
+          launch_ssplit_d3q19<float>(c->popA, c->popB, c->nbw, c->S, c->P, c->geo, c->sparams,
                                      c->stream, k);
         else
-          launch_ssplit_d3q19<__half>(c->popA, c->popB, c->nbw, c->S, c->geo, c->sparams,
+          launch_ssplit_d3q19<__half>(c->popA, c->popB, c->nbw, c->S, c->P, c->geo, c->sparams,
                                       c->stream, k);
       };
       if (c->p.sc) {
@@ -748,11 +750,15 @@ int dlbc_gpu_create(dlbc_gpu_ctx **out, int d, int q, int components, const int6
   c->bufY = c->popB;
   if (storage != DLBC_STORAGE_F64 && hipMalloc(&c->d_self, sizeof(BatchLattice)) != hipSuccess)
     return bail(DLBC_ERR_NOMEM);
+  if (storage != DLBC_STORAGE_F64 &&
+      hipMalloc(&c->P, sizeof(float) * components * c->geo.sp) != hipSuccess)
+    return bail(DLBC_ERR_NOMEM);
   if (hipMalloc(&c->S, sizeof(double) * components * c->geo.sp) != hipSuccess)
     return bail(DLBC_ERR_NOMEM);
   if (hipMalloc(&c->mask, c->geo.sp) != hipSuccess) return bail(DLBC_ERR_NOMEM);
   if (hipMalloc(&c->nbw, sizeof(uint32_t) * c->geo.sp) != hipSuccess) return bail(DLBC_ERR_NOMEM);
-  c->dev_bytes = 2 * popb + sizeof(double) * components * c->geo.sp + 5 * c->geo.sp;
+  c->dev_bytes = 2 * popb + sizeof(double) * components * c->geo.sp + 5 * c->geo.sp +
+                 (c->P ? sizeof(float) * components * c->geo.sp : 0);
   if (hipMemsetAsync(c->mask, 0, c->geo.sp, c->stream) != hipSuccess) return bail(DLBC_ERR_HIP);
   if (hipMemsetAsync(c->nbw, 0, sizeof(uint32_t) * c->geo.sp, c->stream) != hipSuccess)
     return bail(DLBC_ERR_HIP);
@@ -1140,6 +1146,7 @@ void dlbc_gpu_destroy(dlbc_gpu_ctx *c) {
   if (c->popA) (void)hipFree(c->popA);
   if (c->popB) (void)hipFree(c->popB);
   if (c->S) (void)hipFree(c->S);
+  if (c->P) (void)hipFree(c->P);
   if (c->mask) (void)hipFree(c->mask);
   if (c->nbw) (void)hipFree(c->nbw);
   if (c->scratch) (void)hipFree(c->scratch);
@@ -1244,6 +1251,7 @@ int batch_prepare(dlbc_batch *b) {
     t[l].pop[1] = c->popB;
     t[l].nbw = c->nbw;
     t[l].S = c->S;
+    t[l].P = c->P;
     t[l].g = c->geo;
     t[l].p = c->sparams;
   }

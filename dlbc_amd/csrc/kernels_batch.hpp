@@ -35,6 +35,7 @@ struct BatchLattice {
   const void *pop[2];
   const uint32_t *nbw;  // solid-neighbour words (used when p.has_solid)
   const double *S;      // density at solid sites (wall force)
+  float *P;             // psi of the split schedule
   Geo g;
   ShiftParams p;
 };
@@ -341,11 +342,10 @@ __global__ void __launch_bounds__(kSplitThreads)
   const int l = batch_block_lattice(first, nlat, b);
   const BatchLattice &B = lats[l];
   const Store *A = static_cast<const Store *>(B.pop[parity]);
-  double *S = const_cast<double *>(B.S);
   if (B.p.has_solid)
-    ssplit_psi_block<Store, true, false>(A, B.nbw, S, B.g, B.p, b - first[l]);
+    ssplit_psi_block<Store, true, false>(A, B.nbw, B.P, B.g, B.p, b - first[l]);
   else
-    ssplit_psi_block<Store, false, false>(A, B.nbw, S, B.g, B.p, b - first[l]);
+    ssplit_psi_block<Store, false, false>(A, B.nbw, B.P, B.g, B.p, b - first[l]);
 }
 
 template <class Store>
@@ -360,9 +360,11 @@ __global__ void __launch_bounds__(kSplitThreads)
   const Store *A = static_cast<const Store *>(B.pop[parity]);
   Store *W = static_cast<Store *>(const_cast<void *>(B.pop[parity ^ 1]));
   if (B.p.has_solid)
-    ssplit_collide_block<Store, true, false>(A, W, B.nbw, B.S, B.g, B.p, b - first[l], xch);
+    ssplit_collide_block<Store, true, false>(A, W, B.nbw, B.S, B.P, B.g, B.p, b - first[l],
+                                             xch);
   else
-    ssplit_collide_block<Store, false, false>(A, W, B.nbw, B.S, B.g, B.p, b - first[l], xch);
+    ssplit_collide_block<Store, false, false>(A, W, B.nbw, B.S, B.P, B.g, B.p, b - first[l],
+                                              xch);
 }
 
 // One step of the blocks [blk0, blk0 + nblk) of a batch on the split schedule.

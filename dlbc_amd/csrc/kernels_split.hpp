@@ -286,9 +286,10 @@ __global__ void __launch_bounds__(kSplitThreads)
 }
 
 // ---- shifted storage (FP32 / FP16 deviations, FP32 arithmetic) -------------
-// The same two passes; psi goes through S (FP64 array, exact for FP32 values)
-// and the per-site arithmetic is shifted_collide_site, as in the tiled kernel,
-// so a lattice gives the same bits under both schedules.
+// The same two passes; psi goes through an FP32 array P (the values are FP32
+// in every schedule) and the per-site arithmetic is shifted_collide_site, as
+// in the tiled kernel, so a lattice gives the same bits under both schedules.
+// S still gives the density at solid neighbours (wall force).
 
 template <class Store, bool SOLID, bool OFF32>
 __device__ inline void ssplit_pull(const Store *__restrict__ Af, const Geo &g, int s, int x,
@@ -309,27 +310,28 @@ __device__ inline void ssplit_pull(const Store *__restrict__ Af, const Geo &g, i
 template <class Store, bool SOLID, bool OFF32>
 __device__ inline void ssplit_psi_block(const Store *__restrict__ A,
                                         const uint32_t *__restrict__ nbw,
-                                        double *__restrict__ S, const Geo &g,
+                                        float *__restrict__ P, const Geo &g,
                                         const ShiftParams &p, int blk) {
   const int w = threadIdx.x >> 6;
   const int f = w & 1;
   const int s = blk * (kSplitThreads / 2) + (w >> 1) * 64 + (threadIdx.x & 63);
   if (s >= g.sites) return;
   const uint32_t sw = SOLID ? nbw[s] : 0u;
-  if (sw >> 31) return;  // solid: S keeps the site's density
+  if (sw >> 31) return;  // solid: psi is not read there
   int x, y, z;
   coords(s, g, x, y, z);
   float n[19];
   ssplit_pull<Store, SOLID, OFF32>(A + (int64_t)f * 19 * g.sp, g, s, x, y, z, sw, n);
   const float rho0 = f ? p.rho0[1] : p.rho0[0];
-  *elem<OFF32>(S, (int64_t)f * g.sp + s) = (double)psi_f(p.psi_form, rho0 + shifted_drho(n));
+  *elem<OFF32>(P, (int64_t)f * g.sp + s) = psi_f(p.psi_form, rho0 + shifted_drho(n));
 }
 
 // Pass 2 for block blk of a lattice; xch: LDS for the partner exchange.
 template <class Store, bool SOLID, bool OFF32>
 __device__ inline void ssplit_collide_block(const Store *__restrict__ A, Store *__restrict__ B,
                                             const uint32_t *__restrict__ nbw,
-                                            const double *__restrict__ S, const Geo &g,
+                                            const double *__restrict__ S,
+                                            const float *__restrict__ P, const Geo &g,
                                             const ShiftParams &p, int blk,
                                             float (*xch)[4][kSplitThreads / 2]) {
   using L = D3Q19;
@@ -366,7 +368,7 @@ __device__ inline void ssplit_collide_block(const Store *__restrict__ A, Store *
   Store *__restrict__ Bf = B + (int64_t)f * 19 * g.sp;
   shifted_collide_site<SOLID>(
       f, n, rho, rho_p, j, j_p, sw, p, pl,
-      [&](int ff, int i) { return (float)at<OFF32>(S, ff * g.sp + (i < 0 ? s : nbr(i))); },
+      [&](int ff, int i) { return at<OFF32>(P, ff * g.sp + (i < 0 ? s : nbr(i))); },
       [&](int i) { return (float)at<OFF32>(S, f * g.sp + nbr(i)); },
       [&](int i, float v) { store_nt<Store>(elem<OFF32>(Bf, (int64_t)i * g.sp + s), v); });
 }
@@ -374,52 +376,52 @@ __device__ inline void ssplit_collide_block(const Store *__restrict__ A, Store *
 template <class Store, bool SOLID, bool OFF32>
 __global__ void __launch_bounds__(kSplitThreads)
     k_ssplit_psi(const Store *__restrict__ A, const uint32_t *__restrict__ nbw,
-                 double *__restrict__ S, Geo g, ShiftParams p) {
-  ssplit_psi_block<Store, SOLID, OFF32>(A, nbw, S, g, p, blockIdx.x);
+                 float *__restrict__ P, Geo g, ShiftParams p) {
+  ssplit_psi_block<Store, SOLID, OFF32>(A, nbw, P, g, p, blockIdx.x);
 }
 
 template <class Store, bool SOLID, bool OFF32>
 __global__ void __launch_bounds__(kSplitThreads)
     k_ssplit_collide(const Store *__restrict__ A, Store *__restrict__ B,
-                     const uint32_t *__restrict__ nbw, const double *__restrict__ S, Geo g,
-                     ShiftParams p) {
+                     const uint32_t *__restrict__ nbw, const double *__restrict__ S,
+                     const float *__restrict__ P, Geo g, ShiftParams p) {
   __shared__ float xch[2][4][kSplitThreads / 2];
-  ssplit_collide_block<Store, SOLID, OFF32>(A, B, nbw, S, g, p, blockIdx.x, xch);
+  ssplit_collide_block<Store, SOLID, OFF32>(A, B, nbw, S, P, g, p, blockIdx.x, xch);
 }
 
 template <class Store, bool OFF32>
-inline void launch_ssplit_impl(const Store *a, Store *b, const uint32_t *nbw, double *S,
-                               const Geo &g, const ShiftParams &p, hipStream_t stream,
-                               int pass) {
+inline void launch_ssplit_impl(const Store *a, Store *b, const uint32_t *nbw, const double *S,
+                               float *P, const Geo &g, const ShiftParams &p,
+                               hipStream_t stream, int pass) {
   const int nb = split_grid(g.sites);
   const dim3 grid(nb), block(kSplitThreads);
   if (pass == 0) {
     if (p.has_solid)
-      hipLaunchKernelGGL((k_ssplit_psi<Store, true, OFF32>), grid, block, 0, stream, a, nbw, S,
+      hipLaunchKernelGGL((k_ssplit_psi<Store, true, OFF32>), grid, block, 0, stream, a, nbw, P,
                          g, p);
     else
-      hipLaunchKernelGGL((k_ssplit_psi<Store, false, OFF32>), grid, block, 0, stream, a, nbw, S,
+      hipLaunchKernelGGL((k_ssplit_psi<Store, false, OFF32>), grid, block, 0, stream, a, nbw, P,
                          g, p);
   } else {
     if (p.has_solid)
       hipLaunchKernelGGL((k_ssplit_collide<Store, true, OFF32>), grid, block, 0, stream, a, b,
-                         nbw, S, g, p);
+                         nbw, S, P, g, p);
     else
       hipLaunchKernelGGL((k_ssplit_collide<Store, false, OFF32>), grid, block, 0, stream, a, b,
-                         nbw, S, g, p);
+                         nbw, S, P, g, p);
   }
 }
 
 template <class Store>
-inline void launch_ssplit_d3q19(const void *A, void *B, const uint32_t *nbw, double *S,
-                                const Geo &g, const ShiftParams &p, hipStream_t stream,
-                                int pass) {
+inline void launch_ssplit_d3q19(const void *A, void *B, const uint32_t *nbw, const double *S,
+                                float *P, const Geo &g, const ShiftParams &p,
+                                hipStream_t stream, int pass) {
   const Store *a = static_cast<const Store *>(A);
   Store *b = static_cast<Store *>(B);
   if (split_off32(g))
-    launch_ssplit_impl<Store, true>(a, b, nbw, S, g, p, stream, pass);
+    launch_ssplit_impl<Store, true>(a, b, nbw, S, P, g, p, stream, pass);
   else
-    launch_ssplit_impl<Store, false>(a, b, nbw, S, g, p, stream, pass);
+    launch_ssplit_impl<Store, false>(a, b, nbw, S, P, g, p, stream, pass);
 }
 
 template <bool OFF32>
