@@ -1342,8 +1342,8 @@ int dlbc_batch_create(dlbc_batch **out, dlbc_gpu_ctx *const *lattices, int n) {
   if (hipMalloc(&b->d_lats, sizeof(BatchLattice) * n) != hipSuccess) return bail(DLBC_ERR_NOMEM);
   b->h_first.assign(1, 0);
   for (int l = 0; l < n; ++l)
-    b->h_first.push_back(b->h_first.back() + (lattices[l]->geo.sites + kSplitThreads / 2 - 1) /
-                                                 (kSplitThreads / 2));
+    b->h_first.push_back(b->h_first.back() + (lattices[l]->geo.sites + kShiftThreads / 2 - 1) /
+                                                 (kShiftThreads / 2));
   if (hipMalloc(&b->d_first, sizeof(int) * (n + 1)) != hipSuccess) return bail(DLBC_ERR_NOMEM);
   if (hipMemcpy(b->d_first, b->h_first.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice) !=
       hipSuccess)

@@ -334,7 +334,7 @@ __device__ inline int batch_block_lattice(const int *__restrict__ first, int nla
 }
 
 template <class Store>
-__global__ void __launch_bounds__(kSplitThreads)
+__global__ void __launch_bounds__(kShiftThreads)
     k_batch_split_psi(const BatchLattice *__restrict__ lats, const int *__restrict__ first,
                       int nlat, int blk0, int blk1, int parity) {
   const int b = blk0 + blockIdx.x;
@@ -349,10 +349,10 @@ __global__ void __launch_bounds__(kSplitThreads)
 }
 
 template <class Store>
-__global__ void __launch_bounds__(kSplitThreads)
+__global__ void __launch_bounds__(kShiftThreads)
     k_batch_split_collide(const BatchLattice *__restrict__ lats, const int *__restrict__ first,
                           int nlat, int blk0, int blk1, int parity) {
-  __shared__ float xch[2][4][kSplitThreads / 2];
+  __shared__ float xch[2][4][kShiftThreads / 2];
   const int b = blk0 + blockIdx.x;
   if (b >= blk1) return;  // whole block: no barrier is left waiting
   const int l = batch_block_lattice(first, nlat, b);
@@ -373,9 +373,9 @@ inline void launch_batch_split(const BatchLattice *lats, const int *first, int n
                                int nblk, int parity, hipStream_t stream) {
   if (nblk <= 0) return;
   const int grid = nblk;
-  hipLaunchKernelGGL((k_batch_split_psi<Store>), dim3(grid), dim3(kSplitThreads), 0, stream,
+  hipLaunchKernelGGL((k_batch_split_psi<Store>), dim3(grid), dim3(kShiftThreads), 0, stream,
                      lats, first, nlat, blk0, blk0 + nblk, parity);
-  hipLaunchKernelGGL((k_batch_split_collide<Store>), dim3(grid), dim3(kSplitThreads), 0, stream,
+  hipLaunchKernelGGL((k_batch_split_collide<Store>), dim3(grid), dim3(kShiftThreads), 0, stream,
                      lats, first, nlat, blk0, blk0 + nblk, parity);
 }
 

@@ -25,13 +25,16 @@
 
 namespace dlbc {
 
-constexpr int kSplitThreads = 256;  // 4 waves = 128 sites x 2 components
+constexpr int kSplitThreads = 256;  // FP64: 4 waves = 128 sites x 2 components
+// Shifted storage: 8 waves = 256 sites x 2 components (measured: FP32 1.85 ms
+// per 256^3 step against 1.92 ms with 4 waves; FP64 prefers 4: 3.41 vs 3.50).
+constexpr int kShiftThreads = 512;
 
 // Grid of a split launch over `sites`: 128-site blocks, in site order.  (A
 // grid reordered so that each XCD walks one contiguous run of sites measured
 // slower: 3.76 against 3.43 ms per FP64 step at 256^3.)
-inline int split_grid(int64_t sites) {
-  return (int)((sites + kSplitThreads / 2 - 1) / (kSplitThreads / 2));
+inline int split_grid(int64_t sites, int threads = kSplitThreads) {
+  return (int)((sites + threads / 2 - 1) / (threads / 2));
 }
 
 // Site and component of this lane (the component is uniform per wave); false
@@ -314,7 +317,7 @@ __device__ inline void ssplit_psi_block(const Store *__restrict__ A,
                                         const ShiftParams &p, int blk) {
   const int w = threadIdx.x >> 6;
   const int f = w & 1;
-  const int s = blk * (kSplitThreads / 2) + (w >> 1) * 64 + (threadIdx.x & 63);
+  const int s = blk * (kShiftThreads / 2) + (w >> 1) * 64 + (threadIdx.x & 63);
   if (s >= g.sites) return;
   const uint32_t sw = SOLID ? nbw[s] : 0u;
   if (sw >> 31) return;  // solid: psi is not read there
@@ -333,11 +336,11 @@ __device__ inline void ssplit_collide_block(const Store *__restrict__ A, Store *
                                             const double *__restrict__ S,
                                             const float *__restrict__ P, const Geo &g,
                                             const ShiftParams &p, int blk,
-                                            float (*xch)[4][kSplitThreads / 2]) {
+                                            float (*xch)[4][kShiftThreads / 2]) {
   using L = D3Q19;
   const int w = threadIdx.x >> 6;
   const int f = w & 1;
-  const int s = blk * (kSplitThreads / 2) + (w >> 1) * 64 + (threadIdx.x & 63);
+  const int s = blk * (kShiftThreads / 2) + (w >> 1) * 64 + (threadIdx.x & 63);
   const int k = ((threadIdx.x >> 7) << 6) | (threadIdx.x & 63);
   const uint32_t sw = s >= g.sites ? (1u << 31) : (SOLID ? nbw[s] : 0u);
   const ShiftLane pl = shift_lane(p, f);
@@ -374,18 +377,18 @@ __device__ inline void ssplit_collide_block(const Store *__restrict__ A, Store *
 }
 
 template <class Store, bool SOLID, bool OFF32>
-__global__ void __launch_bounds__(kSplitThreads)
+__global__ void __launch_bounds__(kShiftThreads)
     k_ssplit_psi(const Store *__restrict__ A, const uint32_t *__restrict__ nbw,
                  float *__restrict__ P, Geo g, ShiftParams p) {
   ssplit_psi_block<Store, SOLID, OFF32>(A, nbw, P, g, p, blockIdx.x);
 }
 
 template <class Store, bool SOLID, bool OFF32>
-__global__ void __launch_bounds__(kSplitThreads)
+__global__ void __launch_bounds__(kShiftThreads)
     k_ssplit_collide(const Store *__restrict__ A, Store *__restrict__ B,
                      const uint32_t *__restrict__ nbw, const double *__restrict__ S,
                      const float *__restrict__ P, Geo g, ShiftParams p) {
-  __shared__ float xch[2][4][kSplitThreads / 2];
+  __shared__ float xch[2][4][kShiftThreads / 2];
   ssplit_collide_block<Store, SOLID, OFF32>(A, B, nbw, S, P, g, p, blockIdx.x, xch);
 }
 
@@ -393,8 +396,8 @@ template <class Store, bool OFF32>
 inline void launch_ssplit_impl(const Store *a, Store *b, const uint32_t *nbw, const double *S,
                                float *P, const Geo &g, const ShiftParams &p,
                                hipStream_t stream, int pass) {
-  const int nb = split_grid(g.sites);
-  const dim3 grid(nb), block(kSplitThreads);
+  const int nb = split_grid(g.sites, kShiftThreads);
+  const dim3 grid(nb), block(kShiftThreads);
   if (pass == 0) {
     if (p.has_solid)
       hipLaunchKernelGGL((k_ssplit_psi<Store, true, OFF32>), grid, block, 0, stream, a, nbw, P,
