@@ -13,6 +13,7 @@
 #include "../../include/dlbc_hip.h"
 #include "kernels_batch.hpp"
 #include "kernels_fused.hpp"
+#include "kernels_init.hpp"
 #include "kernels_shifted.hpp"
 #include "kernels_split.hpp"
 #include "kernels_twopass.hpp"
@@ -129,13 +130,7 @@ __global__ void k_aos_to_soa(const double *__restrict__ stg, St *__restrict__ A,
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= cnt) return;
   for (int i = 0; i < q; ++i) {
-    double v = stg[(int64_t)t * q + i];
-    St sv;
-    if constexpr (std::is_same<St, double>::value) {
-      sv = v;
-    } else {
-      sv = from_f<St>((float)(v - D3Q19::w[i] * rho0));
-    }
+    const St sv = to_storage<St>(stg[(int64_t)t * q + i], i, rho0);
     A[(int64_t)(f * q + i) * sp + s0 + t] = sv;
     B[(int64_t)(f * q + i) * sp + s0 + t] = sv;
   }
@@ -822,6 +817,8 @@ int dlbc_gpu_set_params(dlbc_gpu_ctx *c, const double *tau, const double *gcc, c
   return DLBC_OK;
 }
 
+static int mask_ready(dlbc_gpu_ctx *c, int solid);
+
 int dlbc_gpu_upload_mask(dlbc_gpu_ctx *c, const int32_t *host, int halo) {
   if (!c || !host || halo < 0) return DLBC_ERR_ARG;
   std::vector<uint8_t> m((size_t)c->geo.sites);
@@ -836,6 +833,12 @@ int dlbc_gpu_upload_mask(dlbc_gpu_ctx *c, const int32_t *host, int halo) {
         solid |= v;
       }
   HIPCHK(c, hipMemcpyAsync(c->mask, m.data(), m.size(), hipMemcpyHostToDevice, c->stream));
+  return mask_ready(c, solid);
+}
+
+// The mask bytes are on the device: derive the solid-neighbour words and the
+// solid flag every step kernel reads.
+static int mask_ready(dlbc_gpu_ctx *c, int solid) {
   {
     const int nb = blocks_for(c->geo.sites, 256);
     if (c->d == 3)
@@ -894,6 +897,155 @@ int dlbc_gpu_upload_populations(dlbc_gpu_ctx *c, int f, const double *host, int 
     HIPCHK(c, hipStreamSynchronize(c->stream));
   }
   c->pops_uploaded[f] = true;
+  c->solid_rho_dirty = true;
+  return DLBC_OK;
+}
+
+// Launch k_init_fluid over sites s0 .. s0+cnt-1 in the lattice's storage.
+static int launch_init_fluid(dlbc_gpu_ctx *c, int s0, int cnt, const double *vals,
+                             const FluidInitArgs &a, const ShapeSpec &sh) {
+  const dim3 grid(blocks_for(cnt, 256)), block(256);
+  if (c->storage == DLBC_STORAGE_F64)
+    hipLaunchKernelGGL(k_init_fluid<double>, grid, block, 0, c->stream,
+                       static_cast<double *>(c->popA), static_cast<double *>(c->popB), c->geo, s0,
+                       cnt, vals, a, sh);
+  else if (c->storage == DLBC_STORAGE_F32_SHIFT)
+    hipLaunchKernelGGL(k_init_fluid<float>, grid, block, 0, c->stream,
+                       static_cast<float *>(c->popA), static_cast<float *>(c->popB), c->geo, s0,
+                       cnt, vals, a, sh);
+  else
+    hipLaunchKernelGGL(k_init_fluid<__half>, grid, block, 0, c->stream,
+                       static_cast<__half *>(c->popA), static_cast<__half *>(c->popB), c->geo, s0,
+                       cnt, vals, a, sh);
+  HIPCHK(c, hipGetLastError());
+  return DLBC_OK;
+}
+
+int dlbc_gpu_init_fluid(dlbc_gpu_ctx *c, int f, int halo, int eq_dist_form, int kind,
+                        const double *dens, int ndens, double perturb, dlbc_rng *rng,
+                        double radius, const double *offset, const double *separation, int axis,
+                        double thickness, const double *widths) {
+  if (!c || f < 0 || f >= c->C || halo < 0) return DLBC_ERR_ARG;
+  FluidInitArgs a{};
+  a.q = c->q;
+  a.f = f;
+  a.halo = halo;
+  a.rho0 = c->rho0[f];
+  if (!eq_unity(c->d, c->q, eq_dist_form, a.u))
+    return fail(c, DLBC_ERR_ARG, "eq_dist_form %d is not a known form", eq_dist_form);
+  if (kind == DLBC_INIT_NONE || kind == DLBC_INIT_FILE) return DLBC_OK;  // nothing is written
+  ShapeSpec sh{};
+  const bool random = kind == DLBC_INIT_CONST_RANDOM || kind == DLBC_INIT_EQDIST_RANDOM ||
+                      kind == DLBC_INIT_EQDIST_PERTURB || kind == DLBC_INIT_EQDIST_PERTURB_FRAC;
+  if (kind == DLBC_INIT_CONST || kind == DLBC_INIT_EQDIST) {  // init.d:26-37, 52-58
+    if (!dens || ndens < 1) return fail(c, DLBC_ERR_ARG, "fluid init needs a density");
+    a.mode = kInitConst;
+    a.value = dens[0];
+    if (kind == DLBC_INIT_CONST)
+      for (int i = 0; i < a.q; ++i) a.u[i] = 1.0;
+  } else if (random) {  // init.d:39-50, 60-82
+    if (!dens || ndens < 1 || !rng) return fail(c, DLBC_ERR_ARG, "random fluid init needs a "
+                                                                 "density and a generator");
+    a.mode = kind == DLBC_INIT_CONST_RANDOM ? kInitPop : kInitSite;
+  } else if (is_shape_kind(kind)) {
+    const int64_t n64[3] = {c->n[0], c->n[1], c->n[2]};
+    if (!resolve_shape(sh, kind, c->d, n64, halo, dens, ndens, radius, offset, separation, axis,
+                       thickness, widths))
+      return fail(c, DLBC_ERR_ARG, "invalid shape arguments for fluid init kind %d", kind);
+    a.mode = kInitShape;
+  } else {
+    return fail(c, DLBC_ERR_ARG, "unknown fluid init kind %d", kind);
+  }
+  if (!random) {
+    int rc = launch_init_fluid(c, 0, c->geo.sites, nullptr, a, sh);
+    if (rc) return rc;
+  } else {
+    // The reference draws over its halo-padded field in row-major order
+    // (one draw per site, per population for ConstRandom); the halo's draws
+    // are discarded.  Physical x-planes go to the device in chunks.
+    const int per = a.mode == kInitPop ? a.q : 1;
+    const int64_t h = halo;
+    const int64_t nh[3] = {c->n[0] + 2 * h, c->d >= 2 ? c->n[1] + 2 * h : 1,
+                           c->d >= 3 ? c->n[2] + 2 * h : 1};
+    const int64_t oy = c->d >= 2 ? h : 0, oz = c->d >= 3 ? h : 0;
+    const int plane = c->n[1] * c->n[2];
+    const int xchunk =
+        std::max(1, std::min(c->n[0], (int)((64u << 20) / (sizeof(double) * per * plane))));
+    int rc = ensure_scratch(c, sizeof(double) * (size_t)per * plane * xchunk);
+    if (rc) return rc;
+    std::vector<double> stg((size_t)per * plane * xchunk);
+    size_t o = 0;
+    int x0 = 0;  // first physical plane in the staging buffer
+    for (int64_t px = 0; px < nh[0]; ++px) {
+      const bool xin = px >= h && px < h + c->n[0];
+      for (int64_t py = 0; py < nh[1]; ++py)
+        for (int64_t pz = 0; pz < nh[2]; ++pz) {
+          const bool in = xin && py >= oy && py < oy + c->n[1] && pz >= oz && pz < oz + c->n[2];
+          if (per == 1) {
+            const double v = random_density(kind, dens[0], perturb, rng);
+            if (in) stg[o++] = v;
+          } else {
+            for (int i = 0; i < per; ++i) {
+              const double v = dens[0] * dlbc_rng_uniform(rng, 0.0, 2.0);
+              if (in) stg[o++] = v;
+            }
+          }
+        }
+      const int xdone = (int)(px - h) + 1;  // physical planes staged so far
+      if (xin && (xdone - x0 == xchunk || xdone == c->n[0])) {
+        const int cnt = (xdone - x0) * plane;
+        HIPCHK(c, hipMemcpyAsync(c->scratch, stg.data(), sizeof(double) * o,
+                                 hipMemcpyHostToDevice, c->stream));
+        rc = launch_init_fluid(c, x0 * plane, cnt, c->scratch, a, sh);
+        if (rc) return rc;
+        HIPCHK(c, hipStreamSynchronize(c->stream));  // the staging buffer is refilled next
+        o = 0;
+        x0 = xdone;
+      }
+    }
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->pops_uploaded[f] = true;
+  c->solid_rho_dirty = true;
+  return DLBC_OK;
+}
+
+int dlbc_gpu_init_mask(dlbc_gpu_ctx *c, int kind, int axis, int wall_offset,
+                       const int *dim_has_wall, int wall_thickness) {
+  if (!c) return DLBC_ERR_ARG;
+  const int64_t n64[3] = {c->n[0], c->n[1], c->n[2]};
+  MaskSpec m;
+  if (!resolve_mask(m, c->d, n64, 0, kind, axis, wall_offset, dim_has_wall, wall_thickness))
+    return fail(c, DLBC_ERR_ARG, "invalid mask init (kind %d, axis %d)", kind, axis);
+  hipLaunchKernelGGL(k_init_mask, dim3(blocks_for(c->geo.sites, 256)), dim3(256), 0, c->stream,
+                     c->mask, c->geo, m);
+  HIPCHK(c, hipGetLastError());
+  std::vector<uint8_t> back((size_t)c->geo.sites);
+  HIPCHK(c, hipMemcpyAsync(back.data(), c->mask, back.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int solid = 0;
+  for (uint8_t v : back) solid |= v;
+  return mask_ready(c, solid);
+}
+
+int dlbc_gpu_init_eq_dist_wall(dlbc_gpu_ctx *c, int f) {
+  if (!c || f < 0 || f >= c->C) return DLBC_ERR_ARG;
+  if (!c->has_solid) return DLBC_OK;
+  const dim3 grid(blocks_for(c->geo.sites, 256)), block(256);
+  if (c->storage == DLBC_STORAGE_F64)
+    hipLaunchKernelGGL(k_eq_dist_wall<double>, grid, block, 0, c->stream,
+                       static_cast<double *>(c->popA), static_cast<double *>(c->popB), c->mask,
+                       c->geo, c->q, f, c->rho0[f]);
+  else if (c->storage == DLBC_STORAGE_F32_SHIFT)
+    hipLaunchKernelGGL(k_eq_dist_wall<float>, grid, block, 0, c->stream,
+                       static_cast<float *>(c->popA), static_cast<float *>(c->popB), c->mask,
+                       c->geo, c->q, f, c->rho0[f]);
+  else
+    hipLaunchKernelGGL(k_eq_dist_wall<__half>, grid, block, 0, c->stream,
+                       static_cast<__half *>(c->popA), static_cast<__half *>(c->popB), c->mask,
+                       c->geo, c->q, f, c->rho0[f]);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   c->solid_rho_dirty = true;
   return DLBC_OK;
 }

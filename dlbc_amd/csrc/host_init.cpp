@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/dlbc_hip.h"
+#include "init_shapes.hpp"
 #include "lattice.hpp"
 
 namespace {
@@ -41,59 +42,6 @@ uint32_t mt_raw(MT &g) {
 
 MT &as_mt(dlbc_rng *r) { return *reinterpret_cast<MT *>(r->state); }
 
-// ---- connectivity weights and eqDistUnity (src/dlbc/lb/eqdist.d:310-315) ----
-// For SecondOrder/ThirdOrder the unit equilibrium is the weight vector; for
-// BDist2 it is LB3D's constant set (D3Q19: 1/3, 2/36, 1/36; D2Q9: 1/2, 1/12,
-// 1/24); for D1Q3 BDist2 returns the population [1, 0, 0] itself.
-bool unity(int d, int q, int form, double *u) {
-  if (d == 3 && q == 19) {
-    if (form == 2) {
-      const double aN1 = 1.0 / 36.0, aN0 = 1.0 / 3.0;
-      u[0] = aN0 * 1.0;
-      for (int i = 1; i < 7; ++i) u[i] = 2.0 * (aN1 * 1.0 + 0.0);
-      for (int i = 7; i < 19; ++i) u[i] = aN1 * 1.0 + 0.0;
-      for (int i = 0; i < 19; ++i) u[i] *= 1.0;
-    } else {
-      u[0] = 1.0 / 3.0;
-      for (int i = 1; i < 7; ++i) u[i] = 1.0 / 18.0;
-      for (int i = 7; i < 19; ++i) u[i] = 1.0 / 36.0;
-    }
-    return true;
-  }
-  if (d == 2 && q == 9) {
-    if (form == 2) {
-      const double aN1 = 1.0 / 24.0;
-      u[0] = 1.0 / 2.0;
-      for (int i = 1; i < 5; ++i) u[i] = 2.0 * aN1;
-      for (int i = 5; i < 9; ++i) u[i] = aN1;
-    } else {
-      u[0] = 4.0 / 9.0;
-      for (int i = 1; i < 5; ++i) u[i] = 1.0 / 9.0;
-      for (int i = 5; i < 9; ++i) u[i] = 1.0 / 36.0;
-    }
-    return true;
-  }
-  if (d == 1 && q == 3) {
-    if (form == 2) {
-      u[0] = 1.0;
-      u[1] = u[2] = 0.0;
-    } else {
-      u[0] = 4.0 / 6.0;
-      u[1] = u[2] = 1.0 / 6.0;
-    }
-    return true;
-  }
-  return false;
-}
-
-// symmetricLinearTransition (src/dlbc/fields/init.d:346-366)
-double transition(double pos, double width, double negval, double posval) {
-  if (width == 0.0) return pos < 0 ? negval : posval;
-  if (pos < -0.5 * width) return negval;
-  if (pos > 0.5 * width) return posval;
-  return negval * (0.5 - pos / width) + posval * (0.5 + pos / width);
-}
-
 struct Box {
   int d;
   int64_t n[3], nh[3];
@@ -111,13 +59,6 @@ bool make_box(Box &b, int d, const int64_t *n, int halo) {
     if (b.n[i] < 1) return false;
   }
   return true;
-}
-
-// dotProduct (src/dlbc/range.d:57-67)
-double dot(const double *a, int d) {
-  double s = 0;
-  for (int i = 0; i < d; ++i) s += a[i] * a[i];
-  return s;
 }
 
 }  // namespace
@@ -152,7 +93,7 @@ int dlbc_init_fluid(double *host, int d, int q, const int64_t *n, int halo, int 
                     const double *widths) {
   Box b;
   double u[19];
-  if (!host || !make_box(b, d, n, halo) || !unity(d, q, form, u)) return DLBC_ERR_ARG;
+  if (!host || !make_box(b, d, n, halo) || !dlbc::eq_unity(d, q, form, u)) return DLBC_ERR_ARG;
   const int64_t S = b.sites_h();
   auto needs = [&](int k) { return dens && ndens >= k; };
   switch (kind) {
@@ -179,13 +120,7 @@ int dlbc_init_fluid(double *host, int d, int q, const int64_t *n, int halo, int 
     case DLBC_INIT_EQDIST_PERTURB_FRAC: {  // init.d:76-82
       if (!needs(1) || !rng) return DLBC_ERR_ARG;
       for (int64_t s = 0; s < S; ++s) {
-        double a;
-        if (kind == DLBC_INIT_EQDIST_RANDOM)
-          a = dens[0] * dlbc_rng_uniform(rng, 0.0, 2.0);
-        else if (kind == DLBC_INIT_EQDIST_PERTURB)
-          a = dens[0] + dlbc_rng_uniform(rng, -perturb, perturb);
-        else
-          a = dens[0] * (1.0 + dlbc_rng_uniform(rng, -perturb, perturb));
+        const double a = dlbc::random_density(kind, dens[0], perturb, rng);
         for (int i = 0; i < q; ++i) host[s * q + i] = a * u[i];
       }
       return DLBC_OK;
@@ -193,89 +128,17 @@ int dlbc_init_fluid(double *host, int d, int q, const int64_t *n, int halo, int 
     default:
       break;
   }
-  // Shapes (init.d:84-274).  Fractional variants scale by the global lengths.
-  const bool frac = kind == DLBC_INIT_EQDIST_SPHERE_FRAC || kind == DLBC_INIT_EQDIST_TWO_SPHERES_FRAC ||
-                    kind == DLBC_INIT_EQDIST_CYLINDER_FRAC || kind == DLBC_INIT_EQDIST_LAMELLAE_FRAC;
-  double off[3] = {0, 0, 0}, sep[3] = {0, 0, 0}, rad = radius;
-  for (int i = 0; i < d; ++i) {
-    off[i] = offset ? offset[i] : 0.0;
-    sep[i] = separation ? separation[i] : 0.0;
-  }
-  if (frac && (kind == DLBC_INIT_EQDIST_SPHERE_FRAC || kind == DLBC_INIT_EQDIST_TWO_SPHERES_FRAC)) {
-    int64_t small = b.n[0];  // init.d:112-118
-    for (int i = 0; i < d; ++i) small = b.n[i] < small ? b.n[i] : small;
-    rad = radius * (double)small;
-    for (int i = 0; i < d; ++i) {
-      off[i] = off[i] * (double)b.n[i];
-      sep[i] = sep[i] * (double)b.n[i];
-    }
-  } else if (kind == DLBC_INIT_EQDIST_CYLINDER_FRAC) {
-    int64_t small = 0;  // init.d:217-224
-    for (int i = 0; i < d; ++i)
-      if (i != axis && (b.n[i] < small || small == 0)) small = b.n[i];
-    rad = radius * (double)small;
-    for (int i = 0; i < d; ++i) off[i] = off[i] * (double)b.n[i];
-  }
-  std::vector<double> inter;
-  if (kind == DLBC_INIT_EQDIST_LAMELLAE || kind == DLBC_INIT_EQDIST_LAMELLAE_FRAC) {
-    if (axis < 0 || axis >= d || !widths || !needs(1)) return DLBC_ERR_ARG;
-    inter.push_back(0.0);  // calculateInterfaces, init.d:335-344
-    for (int i = 0; i < ndens; ++i)
-      inter.push_back(kind == DLBC_INIT_EQDIST_LAMELLAE_FRAC ? widths[i] * (double)b.n[axis]
-                                                              : widths[i]);
-    for (size_t i = 1; i < inter.size(); ++i) inter[i] += inter[i - 1];
-    for (double &v : inter) v -= 0.5;
-  } else if (!needs(2)) {
+  // Shapes (init.d:84-274)
+  dlbc::ShapeSpec sh;
+  if (!dlbc::resolve_shape(sh, kind, d, n, halo, dens, ndens, radius, offset, separation, axis,
+                           thickness, widths))
     return DLBC_ERR_ARG;
-  }
-  const double hd = (double)b.h;
   int64_t s = 0;
   for (int64_t x = 0; x < b.nh[0]; ++x)
     for (int64_t y = 0; y < b.nh[1]; ++y)
       for (int64_t z = 0; z < b.nh[2]; ++z, ++s) {
         const int64_t p[3] = {x, y, z};
-        double val;
-        if (kind == DLBC_INIT_EQDIST_SPHERE || kind == DLBC_INIT_EQDIST_SPHERE_FRAC) {
-          double o[3];  // init.d:94-102
-          for (int i = 0; i < d; ++i)
-            o[i] = ((double)p[i] - hd) - ((double)b.n[i] * 0.5 + off[i]) + 0.5;
-          val = transition(std::sqrt(dot(o, d)) - rad, thickness, dens[0], dens[1]);
-        } else if (kind == DLBC_INIT_EQDIST_TWO_SPHERES ||
-                   kind == DLBC_INIT_EQDIST_TWO_SPHERES_FRAC) {
-          double o1[3], o2[3];  // init.d:142-153
-          for (int i = 0; i < d; ++i) {
-            const double g = (double)p[i] - hd;
-            o1[i] = g - ((double)b.n[i] * 0.5 + off[i] - 0.5 * sep[i]) + 0.5;
-            o2[i] = g - ((double)b.n[i] * 0.5 + off[i] + 0.5 * sep[i]) + 0.5;
-          }
-          const double r1 = std::sqrt(dot(o1, d)) - rad, r2 = std::sqrt(dot(o2, d)) - rad;
-          val = transition(r2 < r1 ? r2 : r1, thickness, dens[0], dens[1]);
-        } else if (kind == DLBC_INIT_EQDIST_CYLINDER || kind == DLBC_INIT_EQDIST_CYLINDER_FRAC) {
-          double o[3];  // init.d:194-206
-          for (int i = 0; i < d; ++i)
-            o[i] = i == axis ? 0.0 : ((double)p[i] - hd) - ((double)b.n[i] * 0.5 + off[i]) + 0.5;
-          val = transition(std::sqrt(dot(o, d)) - rad, thickness, dens[0], dens[1]);
-        } else if (kind == DLBC_INIT_EQDIST_LAMELLAE || kind == DLBC_INIT_EQDIST_LAMELLAE_FRAC) {
-          const double gp = (double)p[axis] - hd;  // init.d:245-258, 315-333
-          size_t best = 0;
-          double md = std::fabs(inter[0] - gp);
-          for (size_t i = 1; i < inter.size(); ++i) {
-            const double df = std::fabs(inter[i] - gp);
-            if (df < md) {
-              md = df;
-              best = i;
-            }
-          }
-          const double diff = gp - inter[best];
-          if (best == 0)
-            val = transition(diff, thickness, dens[ndens - 1], dens[0]);
-          else if ((int)best == ndens)
-            val = transition(diff, thickness, dens[best - 1], dens[0]);
-          else
-            val = transition(diff, thickness, dens[best - 1], dens[best]);
-        } else {
-          return DLBC_ERR_ARG;
-        }
+        const double val = dlbc::shape_value(sh, p);
         for (int i = 0; i < q; ++i) host[s * q + i] = val * u[i];
       }
   return DLBC_OK;
@@ -284,37 +147,16 @@ int dlbc_init_fluid(double *host, int d, int q, const int64_t *n, int halo, int 
 int dlbc_init_mask(int32_t *host, int d, const int64_t *n, int halo, int kind, int axis,
                    int wall_offset, const int *dim_has_wall, int wall_thickness) {
   Box b;
-  if (!host || !make_box(b, d, n, halo)) return DLBC_ERR_ARG;
-  if ((kind == 2 || kind == 3) && (axis < 0 || axis >= d)) return DLBC_ERR_ARG;
-  if (kind == 5 && !dim_has_wall) return DLBC_ERR_ARG;
+  dlbc::MaskSpec m;
+  if (!host || !make_box(b, d, n, halo) ||
+      !dlbc::resolve_mask(m, d, n, halo, kind, axis, wall_offset, dim_has_wall, wall_thickness))
+    return DLBC_ERR_ARG;
   int64_t s = 0;
   for (int64_t x = 0; x < b.nh[0]; ++x)
     for (int64_t y = 0; y < b.nh[1]; ++y)
       for (int64_t z = 0; z < b.nh[2]; ++z, ++s) {
         const int64_t p[3] = {x, y, z};
-        int32_t e = 0;
-        if (kind == 0) {
-        } else if (kind == 2) {  // Plates, init.d:425-436
-          const int64_t g = p[axis] - b.h;
-          if (g == wall_offset || g == b.n[axis] - 1 - wall_offset) e = 1;
-        } else if (kind == 3 || kind == 4) {  // Tube, Box: init.d:377-412
-          for (int i = 0; i < d; ++i) {
-            const int64_t g = p[i] - b.h;
-            if ((kind == 4 || i != axis) && (g == 0 || g == b.n[i] - 1)) e = 1;
-          }
-        } else if (kind == 5) {  // Walls, init.d:449-467
-          for (int i = 0; i < d; ++i) {
-            const int64_t g = p[i] - b.h;
-            if (dim_has_wall[2 * i] && g < wall_thickness) e = 1;
-            // the reference compares signed with unsigned here: negative
-            // (halo) coordinates pass the upper-wall test too
-            if (dim_has_wall[2 * i + 1] && (uint64_t)g >= (uint64_t)(b.n[i] - wall_thickness))
-              e = 1;
-          }
-        } else {
-          return DLBC_ERR_ARG;
-        }
-        host[s] = e;
+        host[s] = dlbc::mask_value(m, p);
       }
   return DLBC_OK;
 }

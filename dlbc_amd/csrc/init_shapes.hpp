@@ -1,0 +1,264 @@
+// init_shapes.hpp -- the reference's initial fluid shapes and masks, one site
+// at a time.
+//
+// The host initialisation (host_init.cpp, into halo-padded host arrays) and
+// the device initialisation (kernels_init.hpp, straight into the device
+// populations) call the same functions here, so both give the same bits.
+// Coordinates are halo-padded: p = physical + halo, as the reference loops
+// over its stored field (src/dlbc/fields/init.d, src/dlbc/lb/init.d).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "../../include/dlbc_hip.h"
+
+namespace dlbc {
+
+// Densities and lamella interfaces one shape carries (device argument size).
+constexpr int kMaxShapeValues = 64;
+
+// One initFluid call of a shape kind, resolved (fractions scaled, lamella
+// interfaces summed) so that a site's value is a pure function of p.
+struct ShapeSpec {
+  int kind, d, axis, ndens, ninter;
+  double nl[3];    // global lengths, as double
+  double hd;       // halo, as double
+  double off[3], sep[3];
+  double rad, thickness;
+  double dens[kMaxShapeValues];
+  double inter[kMaxShapeValues + 1];
+};
+
+// eqDistUnity (src/dlbc/lb/eqdist.d:310-315).  For SecondOrder/ThirdOrder the
+// unit equilibrium is the weight vector; for BDist2 it is LB3D's constant set
+// (D3Q19: 1/3, 2/36, 1/36; D2Q9: 1/2, 1/12, 1/24); for D1Q3 BDist2 returns the
+// population [1, 0, 0] itself.
+inline bool eq_unity(int d, int q, int form, double *u) {
+  if (d == 3 && q == 19) {
+    if (form == 2) {
+      const double aN1 = 1.0 / 36.0, aN0 = 1.0 / 3.0;
+      u[0] = aN0 * 1.0;
+      for (int i = 1; i < 7; ++i) u[i] = 2.0 * (aN1 * 1.0 + 0.0);
+      for (int i = 7; i < 19; ++i) u[i] = aN1 * 1.0 + 0.0;
+      for (int i = 0; i < 19; ++i) u[i] *= 1.0;
+    } else {
+      u[0] = 1.0 / 3.0;
+      for (int i = 1; i < 7; ++i) u[i] = 1.0 / 18.0;
+      for (int i = 7; i < 19; ++i) u[i] = 1.0 / 36.0;
+    }
+    return true;
+  }
+  if (d == 2 && q == 9) {
+    if (form == 2) {
+      const double aN1 = 1.0 / 24.0;
+      u[0] = 1.0 / 2.0;
+      for (int i = 1; i < 5; ++i) u[i] = 2.0 * aN1;
+      for (int i = 5; i < 9; ++i) u[i] = aN1;
+    } else {
+      u[0] = 4.0 / 9.0;
+      for (int i = 1; i < 5; ++i) u[i] = 1.0 / 9.0;
+      for (int i = 5; i < 9; ++i) u[i] = 1.0 / 36.0;
+    }
+    return true;
+  }
+  if (d == 1 && q == 3) {
+    if (form == 2) {
+      u[0] = 1.0;
+      u[1] = u[2] = 0.0;
+    } else {
+      u[0] = 4.0 / 6.0;
+      u[1] = u[2] = 1.0 / 6.0;
+    }
+    return true;
+  }
+  return false;
+}
+
+inline bool is_shape_kind(int kind) {
+  return kind >= DLBC_INIT_EQDIST_SPHERE && kind <= DLBC_INIT_EQDIST_LAMELLAE_FRAC;
+}
+
+// One site's density of the random kinds (src/dlbc/lb/init.d:60-82), one draw.
+inline double random_density(int kind, double dens0, double perturb, dlbc_rng *rng) {
+  if (kind == DLBC_INIT_EQDIST_RANDOM) return dens0 * dlbc_rng_uniform(rng, 0.0, 2.0);
+  if (kind == DLBC_INIT_EQDIST_PERTURB) return dens0 + dlbc_rng_uniform(rng, -perturb, perturb);
+  return dens0 * (1.0 + dlbc_rng_uniform(rng, -perturb, perturb));
+}
+
+// The geometry of a shape kind (src/dlbc/lb/init.d:84-274): fractional
+// variants scale radius, offset and separation by the global lengths
+// (init.d:112-118, 217-224); lamellae sum their widths into interfaces
+// (calculateInterfaces, init.d:335-344).  False on a bad argument.
+inline bool resolve_shape(ShapeSpec &s, int kind, int d, const int64_t *n, int halo,
+                          const double *dens, int ndens, double radius, const double *offset,
+                          const double *separation, int axis, double thickness,
+                          const double *widths) {
+  if (!is_shape_kind(kind) || d < 1 || d > 3 || !dens || ndens < 1 || ndens > kMaxShapeValues)
+    return false;
+  s = ShapeSpec{};
+  s.kind = kind;
+  s.d = d;
+  s.axis = axis;
+  s.ndens = ndens;
+  s.hd = (double)halo;
+  s.thickness = thickness;
+  s.rad = radius;
+  int64_t nn[3];
+  for (int i = 0; i < 3; ++i) {
+    nn[i] = i < d ? n[i] : 1;
+    s.nl[i] = (double)nn[i];
+  }
+  for (int i = 0; i < ndens; ++i) s.dens[i] = dens[i];
+  for (int i = 0; i < d; ++i) {
+    s.off[i] = offset ? offset[i] : 0.0;
+    s.sep[i] = separation ? separation[i] : 0.0;
+  }
+  if (kind == DLBC_INIT_EQDIST_SPHERE_FRAC || kind == DLBC_INIT_EQDIST_TWO_SPHERES_FRAC) {
+    int64_t small = nn[0];
+    for (int i = 0; i < d; ++i) small = nn[i] < small ? nn[i] : small;
+    s.rad = radius * (double)small;
+    for (int i = 0; i < d; ++i) {
+      s.off[i] = s.off[i] * (double)nn[i];
+      s.sep[i] = s.sep[i] * (double)nn[i];
+    }
+  } else if (kind == DLBC_INIT_EQDIST_CYLINDER_FRAC) {
+    int64_t small = 0;
+    for (int i = 0; i < d; ++i)
+      if (i != axis && (nn[i] < small || small == 0)) small = nn[i];
+    s.rad = radius * (double)small;
+    for (int i = 0; i < d; ++i) s.off[i] = s.off[i] * (double)nn[i];
+  }
+  if (kind == DLBC_INIT_EQDIST_LAMELLAE || kind == DLBC_INIT_EQDIST_LAMELLAE_FRAC) {
+    if (axis < 0 || axis >= d || !widths) return false;
+    s.inter[0] = 0.0;
+    for (int i = 0; i < ndens; ++i)
+      s.inter[i + 1] = kind == DLBC_INIT_EQDIST_LAMELLAE_FRAC ? widths[i] * (double)nn[axis]
+                                                               : widths[i];
+    s.ninter = ndens + 1;
+    for (int i = 1; i < s.ninter; ++i) s.inter[i] += s.inter[i - 1];
+    for (int i = 0; i < s.ninter; ++i) s.inter[i] -= 0.5;
+  } else if (ndens < 2) {
+    return false;
+  }
+  return true;
+}
+
+// symmetricLinearTransition (src/dlbc/fields/init.d:346-366)
+__host__ __device__ inline double shape_transition(double pos, double width, double negval,
+                                                   double posval) {
+  if (width == 0.0) return pos < 0 ? negval : posval;
+  if (pos < -0.5 * width) return negval;
+  if (pos > 0.5 * width) return posval;
+  return negval * (0.5 - pos / width) + posval * (0.5 + pos / width);
+}
+
+// dotProduct (src/dlbc/range.d:57-67)
+__host__ __device__ inline double shape_dot(const double *a, int d) {
+  double s = 0;
+  for (int i = 0; i < d; ++i) s += a[i] * a[i];
+  return s;
+}
+
+// The density of a shape at halo-padded site p (src/dlbc/lb/init.d:94-102
+// sphere, 142-153 two spheres, 194-206 cylinder, 245-258 and 315-333
+// lamellae).
+__host__ __device__ inline double shape_value(const ShapeSpec &s, const int64_t *p) {
+  const int d = s.d;
+  const int kind = s.kind;
+  if (kind == DLBC_INIT_EQDIST_SPHERE || kind == DLBC_INIT_EQDIST_SPHERE_FRAC) {
+    double o[3];
+    for (int i = 0; i < d; ++i) o[i] = ((double)p[i] - s.hd) - (s.nl[i] * 0.5 + s.off[i]) + 0.5;
+    return shape_transition(sqrt(shape_dot(o, d)) - s.rad, s.thickness, s.dens[0], s.dens[1]);
+  }
+  if (kind == DLBC_INIT_EQDIST_TWO_SPHERES || kind == DLBC_INIT_EQDIST_TWO_SPHERES_FRAC) {
+    double o1[3], o2[3];
+    for (int i = 0; i < d; ++i) {
+      const double g = (double)p[i] - s.hd;
+      o1[i] = g - (s.nl[i] * 0.5 + s.off[i] - 0.5 * s.sep[i]) + 0.5;
+      o2[i] = g - (s.nl[i] * 0.5 + s.off[i] + 0.5 * s.sep[i]) + 0.5;
+    }
+    const double r1 = sqrt(shape_dot(o1, d)) - s.rad, r2 = sqrt(shape_dot(o2, d)) - s.rad;
+    return shape_transition(r2 < r1 ? r2 : r1, s.thickness, s.dens[0], s.dens[1]);
+  }
+  if (kind == DLBC_INIT_EQDIST_CYLINDER || kind == DLBC_INIT_EQDIST_CYLINDER_FRAC) {
+    double o[3];
+    for (int i = 0; i < d; ++i)
+      o[i] = i == s.axis ? 0.0 : ((double)p[i] - s.hd) - (s.nl[i] * 0.5 + s.off[i]) + 0.5;
+    return shape_transition(sqrt(shape_dot(o, d)) - s.rad, s.thickness, s.dens[0], s.dens[1]);
+  }
+  // lamellae: the nearest interface decides the pair of densities
+  const double gp = (double)p[s.axis] - s.hd;
+  int best = 0;
+  double md = fabs(s.inter[0] - gp);
+  for (int i = 1; i < s.ninter; ++i) {
+    const double df = fabs(s.inter[i] - gp);
+    if (df < md) {
+      md = df;
+      best = i;
+    }
+  }
+  const double diff = gp - s.inter[best];
+  const int nd = s.ndens;
+  if (best == 0) return shape_transition(diff, s.thickness, s.dens[nd - 1], s.dens[0]);
+  if (best == nd) return shape_transition(diff, s.thickness, s.dens[best - 1], s.dens[0]);
+  return shape_transition(diff, s.thickness, s.dens[best - 1], s.dens[best]);
+}
+
+// One initMask call (src/dlbc/lb/mask.d:134-172, shapes src/dlbc/fields/init.d:377-467).
+struct MaskSpec {
+  int kind, d, axis, wall_offset, wall_thickness;
+  int64_t n[3];
+  int64_t h;
+  int dim_has_wall[6];
+};
+
+// The mask value at halo-padded site p: kind 0 None, 2 Plates, 3 Tube, 4 Box,
+// 5 Walls.
+__host__ __device__ inline int32_t mask_value(const MaskSpec &m, const int64_t *p) {
+  const int d = m.d;
+  int32_t e = 0;
+  if (m.kind == 2) {  // Plates, init.d:425-436
+    const int64_t g = p[m.axis] - m.h;
+    if (g == m.wall_offset || g == m.n[m.axis] - 1 - m.wall_offset) e = 1;
+  } else if (m.kind == 3 || m.kind == 4) {  // Tube, Box: init.d:377-412
+    for (int i = 0; i < d; ++i) {
+      const int64_t g = p[i] - m.h;
+      if ((m.kind == 4 || i != m.axis) && (g == 0 || g == m.n[i] - 1)) e = 1;
+    }
+  } else if (m.kind == 5) {  // Walls, init.d:449-467
+    for (int i = 0; i < d; ++i) {
+      const int64_t g = p[i] - m.h;
+      if (m.dim_has_wall[2 * i] && g < m.wall_thickness) e = 1;
+      // the reference compares signed with unsigned here: negative (halo)
+      // coordinates pass the upper-wall test too
+      if (m.dim_has_wall[2 * i + 1] && (uint64_t)g >= (uint64_t)(m.n[i] - m.wall_thickness))
+        e = 1;
+    }
+  }
+  return e;
+}
+
+// Validates an initMask call and fills m; false on a bad argument.
+inline bool resolve_mask(MaskSpec &m, int d, const int64_t *n, int halo, int kind, int axis,
+                         int wall_offset, const int *dim_has_wall, int wall_thickness) {
+  if (d < 1 || d > 3 || halo < 0 || !n) return false;
+  if (kind != 0 && (kind < 2 || kind > 5)) return false;
+  if ((kind == 2 || kind == 3) && (axis < 0 || axis >= d)) return false;
+  if (kind == 5 && !dim_has_wall) return false;
+  m = MaskSpec{};
+  m.kind = kind;
+  m.d = d;
+  m.axis = axis;
+  m.wall_offset = wall_offset;
+  m.wall_thickness = wall_thickness;
+  m.h = halo;
+  for (int i = 0; i < 3; ++i) m.n[i] = i < d ? n[i] : 1;
+  if (kind == 5)
+    for (int i = 0; i < 2 * d; ++i) m.dim_has_wall[i] = dim_has_wall[i];
+  return true;
+}
+
+}  // namespace dlbc

@@ -90,6 +90,10 @@ def lib():
                                 _dp, _dp, i, d, _dp]),
         "dlbc_init_mask": (i, [_ip, i, _i64p, i, i, i, i, _ip, i]),
         "dlbc_init_eq_dist_wall": (i, [_dp, _ip, i, i, _i64p, i]),
+        "dlbc_gpu_init_fluid": (i, [vp, i, i, i, i, _dp, i, d, C.POINTER(_Rng), d, _dp, _dp, i,
+                                    d, _dp]),
+        "dlbc_gpu_init_mask": (i, [vp, i, i, i, _ip, i]),
+        "dlbc_gpu_init_eq_dist_wall": (i, [vp, i]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -124,16 +128,16 @@ def _padded_shape(n, halo):
     return tuple(int(v) + 2 * halo for v in n)
 
 
-def init_fluid(d, q, n, kind, densities, eq_dist_form=0, perturb=0.0, rng=None, radius=0.0,
-               offset=None, separation=None, axis=0, thickness=0.0, widths=None, halo=2):
-    """initFluid (src/dlbc/lb/init.d:263-339) into a new halo-padded array."""
+def _fluid_args(kind, densities, eq_dist_form, perturb, rng, radius, offset, separation, axis,
+                thickness, widths):
+    """The argument list dlbc_init_fluid and dlbc_gpu_init_fluid share, from
+    kind onwards (eq_dist_form first)."""
     if isinstance(kind, str):
         kind = FLUID_INIT[kind]
     if isinstance(axis, str):
         axis = AXIS[axis]
     if isinstance(eq_dist_form, str):
         eq_dist_form = EQ_FORMS[eq_dist_form]
-    out = np.full(_padded_shape(n, halo) + (q,), np.nan)
     dens = np.ascontiguousarray(np.atleast_1d(densities), np.float64)
     off = np.zeros(3) if offset is None else np.ascontiguousarray(
         np.concatenate([np.asarray(offset, np.float64), np.zeros(3)])[:3])
@@ -142,26 +146,39 @@ def init_fluid(d, q, n, kind, densities, eq_dist_form=0, perturb=0.0, rng=None, 
     wid = np.zeros(max(1, dens.size)) if widths is None else np.ascontiguousarray(
         np.asarray(widths, np.float64))
     r = rng.s if rng is not None else _Rng()
-    rc = lib().dlbc_init_fluid(out.reshape(-1), d, q, np.asarray(n, np.int64), halo, eq_dist_form,
-                               kind, dens, dens.size, float(perturb), C.byref(r), float(radius),
-                               off, sep, int(axis), float(thickness), wid)
+    return (int(eq_dist_form), int(kind), dens, dens.size, float(perturb), C.byref(r),
+            float(radius), off, sep, int(axis), float(thickness), wid)
+
+
+def init_fluid(d, q, n, kind, densities, eq_dist_form=0, perturb=0.0, rng=None, radius=0.0,
+               offset=None, separation=None, axis=0, thickness=0.0, widths=None, halo=2):
+    """initFluid (src/dlbc/lb/init.d:263-339) into a new halo-padded array."""
+    out = np.full(_padded_shape(n, halo) + (q,), np.nan)
+    a = _fluid_args(kind, densities, eq_dist_form, perturb, rng, radius, offset, separation, axis,
+                    thickness, widths)
+    rc = lib().dlbc_init_fluid(out.reshape(-1), d, q, np.asarray(n, np.int64), halo, *a)
     if rc != 0:
         raise DlbcError("dlbc_init_fluid: %s" % ERRORS.get(rc, rc))
     return out
 
 
-def init_mask(d, n, kind="None", axis=0, wall_offset=0, dim_has_wall=None, wall_thickness=1,
-              halo=2):
-    """initMask (src/dlbc/lb/mask.d:134-172) into a new halo-padded int32 array."""
+def _mask_args(d, kind, axis, dim_has_wall):
     if isinstance(kind, str):
         kind = MASK_INIT[kind]
     if isinstance(axis, str):
         axis = AXIS[axis]
-    out = np.zeros(_padded_shape(n, halo), np.int32)
     dhw = np.zeros(2 * d, np.int32)
     if dim_has_wall is not None:
         dhw[:] = np.asarray(dim_has_wall, np.int32).reshape(-1)
-    rc = lib().dlbc_init_mask(out.reshape(-1), d, np.asarray(n, np.int64), halo, kind, int(axis),
+    return int(kind), int(axis), dhw
+
+
+def init_mask(d, n, kind="None", axis=0, wall_offset=0, dim_has_wall=None, wall_thickness=1,
+              halo=2):
+    """initMask (src/dlbc/lb/mask.d:134-172) into a new halo-padded int32 array."""
+    kind, axis, dhw = _mask_args(d, kind, axis, dim_has_wall)
+    out = np.zeros(_padded_shape(n, halo), np.int32)
+    rc = lib().dlbc_init_mask(out.reshape(-1), d, np.asarray(n, np.int64), halo, kind, axis,
                               int(wall_offset), dhw, int(wall_thickness))
     if rc != 0:
         raise DlbcError("dlbc_init_mask: %s" % ERRORS.get(rc, rc))
@@ -236,6 +253,27 @@ class Lattice:
         assert p.size == int(np.prod(_padded_shape(self.n, halo))) * self.q
         self._check(lib().dlbc_gpu_upload_populations(self.h, f, p.reshape(-1), halo),
                     "upload_populations")
+
+    def init_fluid(self, f, kind, densities, eq_dist_form=0, perturb=0.0, rng=None, radius=0.0,
+                   offset=None, separation=None, axis=0, thickness=0.0, widths=None, halo=2):
+        """initFluid (src/dlbc/lb/init.d:263-339) of component f computed on
+        the device: the populations of init_fluid() + upload_population(),
+        without the host array.  Random kinds draw from rng in the order of
+        the halo-padded field (halo 2 in the reference)."""
+        a = _fluid_args(kind, densities, eq_dist_form, perturb, rng, radius, offset, separation,
+                        axis, thickness, widths)
+        self._check(lib().dlbc_gpu_init_fluid(self.h, int(f), int(halo), *a), "init_fluid")
+
+    def init_mask(self, kind="None", axis=0, wall_offset=0, dim_has_wall=None,
+                  wall_thickness=1):
+        """initMask (src/dlbc/lb/mask.d:134-172) computed on the device."""
+        kind, axis, dhw = _mask_args(self.d, kind, axis, dim_has_wall)
+        self._check(lib().dlbc_gpu_init_mask(self.h, kind, axis, int(wall_offset), dhw,
+                                             int(wall_thickness)), "init_mask")
+
+    def init_eq_dist_wall(self, f):
+        """initEqDistWall (src/dlbc/fields/init.d:469-477) on the device."""
+        self._check(lib().dlbc_gpu_init_eq_dist_wall(self.h, int(f)), "init_eq_dist_wall")
 
     def step(self, nsteps=1):
         self._check(lib().dlbc_gpu_step(self.h, int(nsteps)), "step")
@@ -318,9 +356,11 @@ class Lattice:
         return lib().dlbc_gpu_device_bytes(self.h)
 
     @classmethod
-    def from_params(cls, p, storage="f64", device=0, schedule=None):
+    def from_params(cls, p, storage="f64", device=0, schedule=None, device_init=True):
         """Start-up sequence of src/main.d:76-147 for a parameter dictionary
-        (same keys as the oracle's Oracle.from_params)."""
+        (same keys as the oracle's Oracle.from_params).  device_init: compute
+        the initial mask and populations on the device (the same bits as the
+        host initialisation and upload, which device_init=False uses)."""
         d, q, nc = p["d"], p["q"], p.get("components", 1)
         n = p["gn"]
         L = cls(d, q, nc, n, storage=storage, device=device)
@@ -329,6 +369,24 @@ class Lattice:
                      p.get("psiForm", "Linear"), p.get("enableShanChen", False), p.get("rho0"))
         if schedule:
             L.set_schedule(schedule)
+        if device_init:
+            if "maskArray" in p:  # MaskInit.File: physical sites from the file
+                L.upload_mask(np.asarray(p["maskArray"], np.int32))
+            else:
+                L.init_mask(p.get("maskInit", "None"), p.get("maskAxis", "X"),
+                            p.get("wallOffset", 0), p.get("dimensionHasWall"),
+                            p.get("wallThickness", 1))
+            grng = Rng(p.get("seed", 0))
+            for f in range(nc):
+                rng = Rng(p["randomSeed"][f]) if p.get("randomReseed", [False] * nc)[f] else grng
+                L.init_fluid(f, p["fluidInit"][f], p["fluidDensities"][f],
+                             p.get("eqDistForm", "SecondOrder"),
+                             p.get("fluidPerturb", [0.0] * nc)[f], rng, p.get("initRadius", 0.0),
+                             p.get("initOffset"), p.get("initSeparation"),
+                             p.get("initAxis", "X"), p.get("interfaceThickness", 0.0),
+                             p.get("lamellaeWidths"))
+                L.init_eq_dist_wall(f)
+            return L
         mask = init_mask(d, n, p.get("maskInit", "None"), p.get("maskAxis", "X"),
                          p.get("wallOffset", 0), p.get("dimensionHasWall"),
                          p.get("wallThickness", 1))

@@ -250,6 +250,29 @@ int dlbc_init_mask(int32_t *host, int d, const int64_t *n, int halo, int kind, i
 int dlbc_init_eq_dist_wall(double *host, const int32_t *mask, int d, int q, const int64_t *n,
                            int halo);
 
+/* Device initialisation: the same initial state written straight into the
+ * device, without the halo-padded host array (8 q B per site) and its upload.
+ *
+ * dlbc_gpu_init_fluid: initFluid (src/dlbc/lb/init.d:263-339) of component f;
+ * the device populations equal dlbc_init_fluid() + dlbc_gpu_upload_populations()
+ * bit for bit.  halo is the halo of the reference's field (2,
+ * src/dlbc/lattice.d:96-103): the random kinds draw from rng in its
+ * halo-padded row-major order (per population for ConstRandom), as
+ * dlbc_init_fluid does, and discard the halo's draws.  Shifted storage takes
+ * rho0 from dlbc_gpu_set_params(), like an upload.  None and File write
+ * nothing. */
+int dlbc_gpu_init_fluid(dlbc_gpu_ctx *ctx, int f, int halo, int eq_dist_form, int kind,
+                        const double *dens, int ndens, double perturb, dlbc_rng *rng,
+                        double radius, const double *offset, const double *separation, int axis,
+                        double thickness, const double *widths);
+/* initMask (src/dlbc/lb/mask.d:134-172, shapes src/dlbc/fields/init.d:377-467) on
+ * the device: equal to dlbc_init_mask() + dlbc_gpu_upload_mask(). */
+int dlbc_gpu_init_mask(dlbc_gpu_ctx *ctx, int kind, int axis, int wall_offset,
+                       const int *dim_has_wall, int wall_thickness);
+/* initEqDistWall (src/dlbc/fields/init.d:469-477) on the device: the solid sites
+ * of component f get [1, 0, ..., 0]. */
+int dlbc_gpu_init_eq_dist_wall(dlbc_gpu_ctx *ctx, int f);
+
 #ifdef __cplusplus
 }
 #endif
