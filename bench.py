@@ -101,12 +101,16 @@ def pmc_traffic(storage, kernel, n):
     return None
 
 
+BLOCK_ORDER = -1  # --block-order: split-schedule XCD block order (-1: the library's choice)
+
+
 def make_lattice(D, p, mask, pops, schedule, storage="f64", rho0=None):
     L = D.Lattice(3, Q, 2, p["gn"], storage=storage)
     L.set_params(p["tau"], p["gcc"], None, None, p["bodyForce"], p["eqDistForm"], p["psiForm"],
                  True, rho0)
     if schedule != "auto":
         L.set_schedule(schedule)
+    L.set_block_order(BLOCK_ORDER)
     L.upload_mask(mask)
     for f in range(2):
         L.upload_population(f, pops[f])
@@ -349,6 +353,8 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--schedule", default="auto", choices=["auto", "two_pass", "fused", "split"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--block-order", type=int, default=-1,
+                    help="split schedule: blocks per XCD turn (0 site order, -1 automatic)")
     ap.add_argument("--no-storage-legs", action="store_true")
     ap.add_argument("--workload", default="channel", choices=["channel", "droplet", "ensemble"],
                     help="channel: the MLUPS line; droplet: shifted-storage errors against "
@@ -375,6 +381,8 @@ def main():
     if args.workload == "ensemble":
         ensemble_workload(D, args.lattices, args.steps)
         return
+    global BLOCK_ORDER
+    BLOCK_ORDER = args.block_order
     n = args.size
     p = channel_params(n)
     t0 = time.perf_counter()
@@ -382,7 +390,8 @@ def main():
     pops = build_inputs(D, p, mask)
     log("inputs %.1f s, solid fraction %.4f" % (time.perf_counter() - t0, mask.mean()))
     L = make_lattice(D, p, mask, pops, args.schedule)
-    log("uploaded; schedule %s; device bytes %.2f GB" % (L.schedule, L.device_bytes() / 1e9))
+    log("uploaded; schedule %s; block order %d; device bytes %.2f GB"
+        % (L.schedule, L.block_order, L.device_bytes() / 1e9))
     m0 = [L.total_mass(f) for f in range(2)]
     dt = time_steps(L, args.steps, args.warmup)
     sites = n ** 3
@@ -413,7 +422,8 @@ def main():
                                "random spheres r 4-12 (solid fraction %.3f), red/blue halves "
                                "1.0/0.05, body force 1e-6 x on component 1, BDist2, tau 1, "
                                "g 3.6, FP64" % (n, n - 1, float(mask.mean())),
-                   "schedule": L.schedule, "sites": sites, "fluid_sites": fluid,
+                   "schedule": L.schedule, "block_order": L.block_order, "sites": sites,
+                   "fluid_sites": fluid,
                    "mass_rel_drift": float("%.3g" % drift)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 1),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",

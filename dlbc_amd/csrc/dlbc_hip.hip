@@ -99,6 +99,22 @@ int ensure_scratch(dlbc_gpu_ctx *c, size_t bytes) {
 
 inline int blocks_for(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
 
+// Block order of the split kernels (split_block): chunk -1 picks it from the
+// geometry, 0 keeps site order, K > 0 gives the XCDs K blocks per turn.  The
+// automatic choice is site order (see kSplitAutoChunk).
+void apply_block_order(dlbc_gpu_ctx *c, int chunk) {
+  const int threads = c->storage == DLBC_STORAGE_F64 ? kSplitThreads : kShiftThreads;
+  const int spb = threads / 2;
+  if (chunk < 0) {
+    chunk = 0;
+    const int64_t plane = (int64_t)c->n[1] * c->n[2];
+    if (kSplitAutoChunk && plane % spb == 0 && (plane / spb) % 8 == 0)
+      chunk = (int)(plane / spb / 8);  // an eighth of an x-plane per turn
+  }
+  c->geo.order_chunk = chunk;
+  c->geo.order_main = split_order_main(split_grid(c->geo.sites, threads), chunk);
+}
+
 // ---- population loaders -----------------------------------------------------
 // A loader returns population i of component f at site s in FP64: the stored
 // value (FP64 storage) or w_i rho0_f + the stored deviation (shifted storage).
@@ -728,6 +744,7 @@ int dlbc_gpu_create(dlbc_gpu_ctx **out, int d, int q, int components, const int6
   c->geo.nz = c->n[2];
   c->geo.sites = (int)sites;
   c->geo.sp = (sites + 63) / 64 * 64;
+  apply_block_order(c, -1);
   auto bail = [&](int code) {
     dlbc_gpu_destroy(c);
     return code;
@@ -1175,6 +1192,14 @@ int dlbc_gpu_set_launch(dlbc_gpu_ctx *c, int min_workgroups) {
   c->target_blocks = min_workgroups;
   return DLBC_OK;
 }
+
+int dlbc_gpu_set_block_order(dlbc_gpu_ctx *c, int chunk) {
+  if (!c || chunk < -1 || chunk > (1 << 20)) return DLBC_ERR_ARG;
+  apply_block_order(c, chunk);
+  return DLBC_OK;
+}
+
+int dlbc_gpu_get_block_order(const dlbc_gpu_ctx *c) { return c ? c->geo.order_chunk : -1; }
 
 int dlbc_gpu_density(dlbc_gpu_ctx *c, int f, double *host, int halo) {
   if (!c || !host || halo < 0 || f < 0 || f >= c->C) return DLBC_ERR_ARG;

@@ -30,19 +30,43 @@ constexpr int kSplitThreads = 256;  // FP64: 4 waves = 128 sites x 2 components
 // per 256^3 step against 1.92 ms with 4 waves; FP64 prefers 4: 3.41 vs 3.50).
 constexpr int kShiftThreads = 512;
 
-// Grid of a split launch over `sites`: 128-site blocks, in site order.  (A
-// grid reordered so that each XCD walks one contiguous run of sites measured
-// slower: 3.76 against 3.43 ms per FP64 step at 256^3.)
+// Grid of a split launch over `sites`: threads / 2 sites per block.  (A grid
+// that gave each XCD one contiguous run of sites, an eighth of the lattice,
+// measured slower: 3.76 against 3.43 ms per FP64 step at 256^3.)
 inline int split_grid(int64_t sites, int threads = kSplitThreads) {
   return (int)((sites + threads / 2 - 1) / (threads / 2));
 }
+
+// The block of sites this workgroup takes.  Workgroup b runs on XCD b % 8.
+// With g.order_chunk = K the XCDs take turns at K consecutive blocks: the
+// j-th workgroup of XCD k takes block ((j / K) 8 + k) K + j % K.  Rows
+// neighbouring in y then share an XCD and its L2, and in x too when a plane
+// is a whole number of turns (8 K blocks).  Blocks from g.order_main on (the
+// part of the grid short of a whole turn) keep site order.
+__device__ inline int split_block(const Geo &g) {
+  const int b = blockIdx.x;
+  const int K = g.order_chunk;
+  if (K <= 0 || b >= g.order_main) return b;
+  const int j = b >> 3;
+  return ((j / K) * 8 + (b & 7)) * K + j % K;
+}
+
+// Whether the library picks an XCD block order by itself (an eighth of an
+// x-plane per turn) or keeps site order.  Site order: on the 256^3 channel
+// (FP64) chunks of 8 to 256 blocks cut the collide pass's L2 misses by a
+// quarter (FETCH_SIZE 3.49 -> 2.60 GiB at K = 64) yet ran 2.53-2.74 ms
+// against 2.46 ms in site order.
+constexpr bool kSplitAutoChunk = false;
+
+// order_main for chunk K over a grid of nb blocks: whole turns of 8 K blocks.
+inline int split_order_main(int nb, int K) { return K > 0 ? nb / (8 * K) * (8 * K) : 0; }
 
 // Site and component of this lane (the component is uniform per wave); false
 // past the end of the lattice.
 __device__ inline bool split_lane(const Geo &g, int &s, int &f) {
   const int w = threadIdx.x >> 6;
   f = w & 1;
-  s = blockIdx.x * (kSplitThreads / 2) + (w >> 1) * 64 + (threadIdx.x & 63);
+  s = split_block(g) * (kSplitThreads / 2) + (w >> 1) * 64 + (threadIdx.x & 63);
   return s < g.sites;
 }
 
@@ -380,7 +404,7 @@ template <class Store, bool SOLID, bool OFF32>
 __global__ void __launch_bounds__(kShiftThreads)
     k_ssplit_psi(const Store *__restrict__ A, const uint32_t *__restrict__ nbw,
                  float *__restrict__ P, Geo g, ShiftParams p) {
-  ssplit_psi_block<Store, SOLID, OFF32>(A, nbw, P, g, p, blockIdx.x);
+  ssplit_psi_block<Store, SOLID, OFF32>(A, nbw, P, g, p, split_block(g));
 }
 
 template <class Store, bool SOLID, bool OFF32>
@@ -389,7 +413,7 @@ __global__ void __launch_bounds__(kShiftThreads)
                      const uint32_t *__restrict__ nbw, const double *__restrict__ S,
                      const float *__restrict__ P, Geo g, ShiftParams p) {
   __shared__ float xch[2][4][kShiftThreads / 2];
-  ssplit_collide_block<Store, SOLID, OFF32>(A, B, nbw, S, P, g, p, blockIdx.x, xch);
+  ssplit_collide_block<Store, SOLID, OFF32>(A, B, nbw, S, P, g, p, split_block(g), xch);
 }
 
 template <class Store, bool OFF32>

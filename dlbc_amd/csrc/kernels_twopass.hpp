@@ -23,6 +23,10 @@ struct Geo {
   int nx, ny, nz;
   int sites;
   int64_t sp;
+  // Block order of the split kernels (kernels_split.hpp split_block): chunk
+  // K > 0 hands the XCDs K consecutive blocks each in turn over the first
+  // order_main blocks; 0 keeps site order.
+  int order_chunk = 0, order_main = 0;
 };
 
 struct StepParams {

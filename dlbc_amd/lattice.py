@@ -62,6 +62,8 @@ def lib():
         "dlbc_gpu_set_schedule": (i, [vp, i]),
         "dlbc_gpu_get_schedule": (i, [vp]),
         "dlbc_gpu_set_launch": (i, [vp, i]),
+        "dlbc_gpu_set_block_order": (i, [vp, i]),
+        "dlbc_gpu_get_block_order": (i, [vp]),
         "dlbc_gpu_laplace": (i, [vp, _dp]),
         "dlbc_batch_create": (i, [C.POINTER(vp), C.POINTER(vp), i]),
         "dlbc_batch_set_config": (i, [vp, i, i, i, i]),
@@ -301,6 +303,14 @@ class Lattice:
     def set_launch(self, min_workgroups):
         """Minimum workgroups per fused-kernel launch (0: default)."""
         self._check(lib().dlbc_gpu_set_launch(self.h, int(min_workgroups)), "set_launch")
+
+    def set_block_order(self, chunk):
+        """Split schedule: K > 0 blocks per XCD turn, 0 site order, -1 automatic."""
+        self._check(lib().dlbc_gpu_set_block_order(self.h, int(chunk)), "set_block_order")
+
+    @property
+    def block_order(self):
+        return lib().dlbc_gpu_get_block_order(self.h)
 
     @property
     def schedule(self):

@@ -131,6 +131,13 @@ int dlbc_gpu_get_schedule(const dlbc_gpu_ctx *ctx);
  * default, one workgroup per CU for FP64 storage and two for the shifted
  * storages).  No counterpart in the reference (its loops are per rank). */
 int dlbc_gpu_set_launch(dlbc_gpu_ctx *ctx, int min_workgroups);
+/* Block order of the split schedule: chunk K > 0 gives the GPU's XCDs K
+ * consecutive blocks of sites each in turn (block b runs on XCD b mod 8), so
+ * neighbouring rows share an XCD's L2; 0 keeps site order; -1 lets the
+ * library choose.  The results do not depend on it.  No counterpart in the
+ * reference. */
+int dlbc_gpu_set_block_order(dlbc_gpu_ctx *ctx, int chunk);
+int dlbc_gpu_get_block_order(const dlbc_gpu_ctx *ctx);
 
 /* Download component f into a host array [n0+2h]..[q]: physical sites are
  * written, halo sites are left untouched (the caller's exchangeHalo fills
