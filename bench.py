@@ -167,8 +167,20 @@ def storage_leg(D, p, mask, pops, storage, steps, warmup, fluid):
     return out
 
 
-def cpu_baseline(p, mask, pops, steps=2):
-    """The oracle (C restatement of the reference's passes) on the host cores."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(p, mask, pops, seconds=10.0, max_steps=60):
+    """The oracle (C restatement of the reference's passes) on the host cores:
+    whole steps of the workload until about `seconds` of wall time."""
     from oracle import oracle as OR
     OR.build()
     n = p["gn"]
@@ -179,12 +191,15 @@ def cpu_baseline(p, mask, pops, steps=2):
         o.set_population(f, pops[f])
     o.exchange_halo()
     o.run(1)  # warm-up (first touch of the AoS fields)
+    steps = 0
     t0 = time.perf_counter()
-    o.run(steps)
+    while steps < max_steps and (steps < 2 or time.perf_counter() - t0 < seconds):
+        o.run(1)
+        steps += 1
     dt = time.perf_counter() - t0
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": round(float(np.prod(n)) * steps / dt / 1e6, 3), "unit": "MLUPS",
-            "cores": threads, "kind": "port",
+            "cores": threads, "kind": "port", "seconds": round(dt, 2), "cpu": cpu_model(),
             "sample": "%d steps of the same %s workload (AoS + halo 2, reference pass "
                       "order, OpenMP); the body force, which the reference adds from a "
                       "plugin hook, is left out of this leg" % (steps, "x".join(map(str, n)))}

@@ -365,6 +365,46 @@ def test_split_block_order_does_not_change_results(D, storage):
             assert np.array_equal(other[f].view(np.uint64), pops[0][f].view(np.uint64))
 
 
+@pytest.mark.parametrize("dq,n,nc", [((3, 19), [7, 6, 5], 3), ((2, 9), [12, 9], 4),
+                                     ((1, 3), [31], 3), ((3, 19), [6, 5, 4], 4)])
+@pytest.mark.parametrize("form", ["SecondOrder", "BDist2"])
+def test_more_than_two_components_bitexact(D, oracle_mod, dq, n, nc, form):
+    """Three and four components (the general two-pass schedule), with
+    self- and cross-couplings, wall couplings and a random mask, against the
+    oracle bit for bit after 25 steps."""
+    d, q = dq
+    rng = np.random.default_rng(nc * 10 + d)
+    mask = (rng.random(n) < 0.1).astype(np.int32)
+    g = rng.uniform(0.5, 2.0, (nc, nc))
+    gcc = ((g + g.T) / 2.0).tolist()
+    p = dict(d=d, q=q, gn=list(n), components=nc, tau=list(rng.uniform(0.8, 1.3, nc)),
+             eqDistForm=form, gcc=gcc, gwc=list(rng.uniform(-0.2, 0.2, nc)),
+             enableShanChen=True, psiForm="Exponential",
+             globalAcc=[1e-5] + [0.0] * (d - 1), fluidInit=["EqDistPerturb"] * nc,
+             fluidDensities=[[v] for v in rng.uniform(0.2, 0.8, nc)],
+             fluidPerturb=[0.02] * nc, randomReseed=[True] * nc,
+             randomSeed=list(range(21, 21 + nc)), maskArray=mask)
+    gl, o = _pair(D, oracle_mod, p)
+    gl.step(25)
+    o.run(25)
+    _same_state(gl, o, "C=%d D%dQ%d %s" % (nc, d, q, form))
+
+
+@pytest.mark.parametrize("n", [[2, 2, 2], [2, 3, 5], [3, 2, 7]])
+@pytest.mark.parametrize("schedule", ["two_pass", "split"])
+def test_smallest_lattices_bitexact(D, oracle_mod, n, schedule):
+    """Lattices two sites wide, where a site's +1 and -1 neighbours are the
+    same site, against the oracle bit for bit."""
+    p = _walls_case(n, "BDist2")
+    p["maskArray"] = np.zeros(n, np.int32)
+    p["maskArray"][0, 0, 0] = 1  # one solid site: bounce-back and wall force
+    g = D.Lattice.from_params(p, schedule=schedule)
+    o = oracle_mod.Oracle.from_params(p)
+    g.step(10)
+    o.run(10)
+    _same_state(g, o, "n=%s %s" % (n, schedule))
+
+
 def _trace_close(got, row, rel=6e-7):
     t, sigma, radius, p_in, p_out, deform = row
     for name, ref in (("sigma", sigma), ("radius", radius), ("in_pressure", p_in),
