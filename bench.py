@@ -167,6 +167,45 @@ def storage_leg(D, p, mask, pops, storage, steps, warmup, fluid):
     return out
 
 
+def storage_accuracy(D, p, mask, pops, storages, steps):
+    """Error of the shifted storages against the FP64 path on the same
+    workload after `steps` steps: density, velocity (max norm relative to the
+    largest FP64 speed, and relative 2-norm over all fluid sites) and mass."""
+    import torch
+    fluid = mask == 0
+
+    def run(storage):
+        L = make_lattice(D, p, mask, pops, "auto", storage,
+                         None if storage == "f64" else [0.525, 0.525])
+        m0 = [L.total_mass(f) for f in range(2)]
+        L.step(steps)
+        out = {"rho": [L.density(f) for f in range(2)],
+               "u": [L.velocity(f) for f in range(2)],
+               "drift": max(abs(L.total_mass(f) - m0[f]) / m0[f] for f in range(2))}
+        del L
+        torch.cuda.empty_cache()
+        return out
+
+    ref = run("f64")
+    res = {"steps": steps}
+    for st in storages:
+        r = run(st)
+        du = [r["u"][f][fluid] - ref["u"][f][fluid] for f in range(2)]
+        uref = [ref["u"][f][fluid] for f in range(2)]
+        umax = max(float(np.abs(u).max()) for u in uref)
+        res[st] = {
+            "max_abs_drho": max(float(np.abs(r["rho"][f] - ref["rho"][f])[fluid].max())
+                                for f in range(2)),
+            "max_abs_du": max(float(np.abs(d).max()) for d in du),
+            "max_u": umax,
+            "max_abs_du_over_max_u": max(float(np.abs(d).max()) for d in du) / umax,
+            "rel_l2_du": max(float(np.linalg.norm(du[f]) / np.linalg.norm(uref[f]))
+                             for f in range(2)),
+            "mass_rel_drift": r["drift"], "f64_mass_rel_drift": ref["drift"]}
+        log("accuracy %s after %d steps: %s" % (st, steps, res[st]))
+    return res
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as fh:
@@ -371,6 +410,9 @@ def main():
     ap.add_argument("--block-order", type=int, default=-1,
                     help="split schedule: blocks per XCD turn (0 site order, -1 automatic)")
     ap.add_argument("--no-storage-legs", action="store_true")
+    ap.add_argument("--accuracy-steps", type=int, default=2000,
+                    help="steps after which the shifted storages are compared with FP64 "
+                         "(0: no comparison)")
     ap.add_argument("--workload", default="channel", choices=["channel", "droplet", "ensemble"],
                     help="channel: the MLUPS line; droplet: shifted-storage errors against "
                          "FP64 on a centred droplet (radius 96 at 512^3, scaled with --size); "
@@ -456,6 +498,9 @@ def main():
             leg = storage_leg(D, p, mask, pops, st, args.steps, args.warmup, fluid)
             log("%s: %s" % (st, leg))
             result["storage_legs"][st] = leg
+    if not args.no_storage_legs and args.accuracy_steps > 0:
+        result["storage_accuracy"] = storage_accuracy(D, p, mask, pops, ("f32_shift", "f16_shift"),
+                                                      args.accuracy_steps)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(p, mask, pops)

@@ -50,7 +50,7 @@ def _diff(L, ref, m0):
 
 
 # bounds per storage: max |d rho|, max |d u|, relative mass drift, droplet volume
-BOUNDS = {"f32_shift": (3e-5, 1e-5, 1e-4, 0.0), "f16_shift": (2.5e-2, 1.3e-2, 3e-3, 0.02)}
+BOUNDS = {"f32_shift": (4e-6, 1.5e-6, 8e-6, 0.0), "f16_shift": (2.5e-2, 1.3e-2, 3e-3, 0.02)}
 
 
 @pytest.mark.parametrize("storage", ["f32_shift", "f16_shift"])
@@ -58,9 +58,10 @@ BOUNDS = {"f32_shift": (3e-5, 1e-5, 1e-4, 0.0), "f16_shift": (2.5e-2, 1.3e-2, 3e
 def test_droplets_1000_steps(D, name, storage):
     """The reference's 64^3 BDist2 Shan-Chen droplet cases, 1000 steps:
     density and velocity fields, mass drift and droplet volume (sites of
-    positive colour) against FP64.  FP32 drifts by up to 5e-5 in mass over
-    1000 steps: in the steady bulk phases every site repeats the same
-    rounding of its stored deviations each step."""
+    positive colour) against FP64.  FP32 drifts by up to 3.2e-6 in mass over
+    1000 steps.  (With the weights and BDist2 constants rounded to FP32 once,
+    instead of applied as float-float pairs, these differences were ten
+    times larger: 1.4e-5 in density, 5.3e-6 in velocity, 4.8e-5 in mass.)"""
     p = case(name)
     ref, _ = _run(D, p, "f64", 1000)
     L, m0 = _run(D, p, storage, 1000, rho0=[0.37, 0.37])
@@ -94,7 +95,7 @@ def _channel(n):
     return p
 
 
-@pytest.mark.parametrize("storage,drho", [("f32_shift", 1e-4), ("f16_shift", 5e-2)])
+@pytest.mark.parametrize("storage,drho", [("f32_shift", 6e-6), ("f16_shift", 5e-2)])
 def test_channel_walls_body_force(D, storage, drho):
     """The benchmark's channel (walls, random spheres, body force) at 64^3,
     300 steps: finite and close to FP64."""
