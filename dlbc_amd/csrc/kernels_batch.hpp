@@ -361,10 +361,10 @@ __global__ void __launch_bounds__(kShiftThreads)
   Store *W = static_cast<Store *>(const_cast<void *>(B.pop[parity ^ 1]));
   if (B.p.has_solid)
     ssplit_collide_block<Store, true, false>(A, W, B.nbw, B.S, B.P, B.g, B.p, b - first[l],
-                                             xch);
+                                             xch, nullptr);
   else
     ssplit_collide_block<Store, false, false>(A, W, B.nbw, B.S, B.P, B.g, B.p, b - first[l],
-                                              xch);
+                                              xch, nullptr);
 }
 
 // One step of the blocks [blk0, blk0 + nblk) of a batch on the split schedule.

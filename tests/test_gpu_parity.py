@@ -330,12 +330,14 @@ def test_fused_equals_two_pass_with_body_force(D):
         assert_close(c.population(f), b.population(f), 0.0, "split vs two-pass f%d" % f)
 
 
+@pytest.mark.parametrize("n", [[9, 13, 21], [5, 4, 128]], ids=["unaligned", "z-rows"])
 @pytest.mark.parametrize("form", ["SecondOrder", "BDist2", "ThirdOrder"])
 @pytest.mark.parametrize("walls", [False, True], ids=["periodic", "walls"])
-def test_split_schedule_matches_oracle(D, oracle_mod, form, walls):
+def test_split_schedule_matches_oracle(D, oracle_mod, form, walls, n):
     """The lane-per-component two-pass D3Q19 kernels against the oracle, bit
-    for bit, on a lattice no tile divides (any shape is allowed)."""
-    n = [9, 13, 21]
+    for bit: on a lattice no tile divides (any shape is allowed), and on one
+    whose z-rows hold whole blocks, where the collide pass stages the psi rows
+    in LDS."""
     p = _walls_case(n, form) if walls else dict(_config2(8, form), gn=n)
     g = D.Lattice.from_params(p, schedule="split")
     assert g.schedule == "split"
