@@ -379,7 +379,7 @@ def ensemble_workload(D, count, steps, storage="f32_shift"):
                                   "L ~ U{32..96}" % count, "results": res}), flush=True)
 
 
-def copy_calibration(gib=2.0, reps=5):
+def copy_bandwidth(gib=2.0, reps=5):
     """A plain streaming kernel of known traffic (torch.neg over FP64 of
     `gib` GiB: every byte read once and written once), for scaling the
     FETCH_SIZE / WRITE_SIZE counters and as the measured copy bandwidth."""
@@ -394,9 +394,14 @@ def copy_calibration(gib=2.0, reps=5):
         torch.neg(x, out=y)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / reps
-    print(json.dumps({"copy_kernel": "torch.neg fp64", "bytes_read": 8 * n, "bytes_written": 8 * n,
-                      "ms": round(dt * 1e3, 4), "GBps_read_plus_write": round(16 * n / dt / 1e9, 1)}),
-          flush=True)
+    del x, y
+    torch.cuda.empty_cache()
+    return {"copy_kernel": "torch.neg fp64", "bytes_read": 8 * n, "bytes_written": 8 * n,
+            "ms": round(dt * 1e3, 4), "GBps_read_plus_write": round(16 * n / dt / 1e9, 1)}
+
+
+def copy_calibration():
+    print(json.dumps(copy_bandwidth()), flush=True)
 
 
 def main():
@@ -492,6 +497,10 @@ def main():
     }
     del L
     torch.cuda.empty_cache()
+    # the practical ceiling of a streaming kernel on this GPU, beside the nominal peak
+    cb = copy_bandwidth()
+    result["roofline"]["copy_GBps"] = cb["GBps_read_plus_write"]
+    result["roofline"]["frac_of_copy"] = round(achieved / 1e9 / cb["GBps_read_plus_write"], 4)
     if not args.no_storage_legs:
         result["storage_legs"] = {}
         for st in ("f32_shift", "f16_shift"):
