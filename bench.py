@@ -321,6 +321,38 @@ def droplet_workload(D, n, steps, storages=("f16_shift", "f32_shift")):
                       "init_seconds": init_s, "results": result}), flush=True)
 
 
+def slab_workload(D, n, planes, steps, warmup):
+    """One GPU's share of the channel when x is cut into slabs: the first
+    `planes` x-planes of the n^3 workload, periodic in x (the wrap stands in
+    for the face exchange). Throughput per storage, and the bytes a face
+    exchange would move per step (the populations crossing the face plus one
+    psi plane, both components)."""
+    import torch
+    p = channel_params(n)
+    mask = channel_mask(n)
+    pops = build_inputs(D, p, mask)
+    mask = np.ascontiguousarray(mask[:planes])
+    pops = [np.ascontiguousarray(pp[:planes]) for pp in pops]
+    ps = dict(p, gn=[planes, n, n])
+    sites = planes * n * n
+    out = {}
+    for st in ("f64", "f32_shift", "f16_shift"):
+        L = make_lattice(D, ps, mask, pops, "auto", st, None if st == "f64" else [0.525, 0.525])
+        dt = time_steps(L, steps, warmup)
+        el, kern = L.step_timed(max(5, steps // 2))
+        elem = {"f64": 8, "f32_shift": 4, "f16_shift": 2}[st]
+        out[st] = {"mlups": round(sites * steps / dt / 1e6, 1),
+                   "ms_per_step": round(1e3 * dt / steps, 4),
+                   "kernels_ms": {k: round(v, 4) for k, v in kern.items()},
+                   "face_exchange_bytes": 2 * n * n * (5 * C * elem + C * 8)}
+        log("slab %s: %s" % (st, out[st]))
+        del L
+        torch.cuda.empty_cache()
+    print(json.dumps({"workload": "%dx%dx%d slab of the %d^3 channel (one GPU's share of %d)"
+                                  % (planes, n, n, n, n // planes),
+                      "steps": steps, "results": out}), flush=True)
+
+
 def ensemble_params(count=256, seed=20261202):
     """Independent droplet lattices: cubic L ~ U{32..96}, tau ~ U[1.0, 1.5]
     (both components), g ~ U[3.0, 4.5], radius ~ U[0.2, 0.35] * L
@@ -418,11 +450,15 @@ def main():
     ap.add_argument("--accuracy-steps", type=int, default=2000,
                     help="steps after which the shifted storages are compared with FP64 "
                          "(0: no comparison)")
-    ap.add_argument("--workload", default="channel", choices=["channel", "droplet", "ensemble"],
+    ap.add_argument("--workload", default="channel",
+                    choices=["channel", "droplet", "ensemble", "slab"],
                     help="channel: the MLUPS line; droplet: shifted-storage errors against "
                          "FP64 on a centred droplet (radius 96 at 512^3, scaled with --size); "
-                         "ensemble: 256 uneven lattices, batched vs looped")
+                         "ensemble: 256 uneven lattices, batched vs looped; slab: one GPU's "
+                         "x-slab of the channel")
     ap.add_argument("--lattices", type=int, default=256, help="ensemble size")
+    ap.add_argument("--slab-planes", type=int, default=32,
+                    help="slab workload: x-planes of one GPU's share")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="time a plain FP64 streaming kernel of known size and exit")
     args = ap.parse_args()
@@ -442,6 +478,9 @@ def main():
         return
     if args.workload == "ensemble":
         ensemble_workload(D, args.lattices, args.steps)
+        return
+    if args.workload == "slab":
+        slab_workload(D, args.size, args.slab_planes, args.steps, args.warmup)
         return
     global BLOCK_ORDER
     BLOCK_ORDER = args.block_order
