@@ -236,9 +236,17 @@ def cpu_baseline(p, mask, pops, seconds=10.0, max_steps=60):
         o.run(1)
         steps += 1
     dt = time.perf_counter() - t0
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = OR.lib().or_get_threads()
+    # one step on one core, for the single-core rate
+    OR.lib().or_set_threads(1)
+    t1 = time.perf_counter()
+    o.run(1)
+    dt1 = time.perf_counter() - t1
+    OR.lib().or_set_threads(threads)
     return {"value": round(float(np.prod(n)) * steps / dt / 1e6, 3), "unit": "MLUPS",
             "cores": threads, "kind": "port", "seconds": round(dt, 2), "cpu": cpu_model(),
+            "nproc": os.cpu_count(),
+            "single_core_value": round(float(np.prod(n)) / dt1 / 1e6, 3),
             "sample": "%d steps of the same %s workload (AoS + halo 2, reference pass "
                       "order, OpenMP); the body force, which the reference adds from a "
                       "plugin hook, is left out of this leg" % (steps, "x".join(map(str, n)))}

@@ -16,6 +16,9 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define OR_MAXQ 19
 #define OR_MAXC 8
@@ -1206,4 +1209,20 @@ int or_init_mask(or_lattice *L, int kind, int axis, int wall_offset, const int *
         L->mask[idx3(L, x, y, z)] = e;
       }
   return 0;
+}
+
+void or_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+int or_get_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
 }

@@ -129,6 +129,10 @@ int or_init_mask(or_lattice *L, int kind, int axis, int wall_offset,
                  const int *dim_has_wall /*d*2*/, int wall_thickness);
 void or_init_eq_dist_wall(or_lattice *L, int f);
 
+/* Threads of the OpenMP loops (timing only; results do not depend on it). */
+void or_set_threads(int n);
+int or_get_threads(void);
+
 #ifdef __cplusplus
 }
 #endif

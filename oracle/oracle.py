@@ -75,6 +75,8 @@ def lib():
             "or_run_timesteps": (None, [vp, C.c_int]),
             "or_timers": (None, [vp, _dp]),
             "or_local_total_mass": (C.c_double, [vp, C.c_int]),
+            "or_set_threads": (None, [C.c_int]),
+            "or_get_threads": (C.c_int, []),
             "or_local_total_momentum": (None, [vp, C.c_int, _dp]),
             "or_velocity_field": (None, [vp, C.c_int, _dp]),
             "or_density_field_phys": (None, [vp, C.c_int, _dp]),
