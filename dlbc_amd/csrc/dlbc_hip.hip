@@ -340,6 +340,7 @@ int dispatch(dlbc_gpu_ctx *c, Args... args) {
   };
   if (c->d == 3) return go(D3Q19{});
   if (c->d == 2) return go(D2Q9{});
+  if (c->q == 5) return go(D1Q5{});
   return go(D1Q3{});
 }
 
@@ -507,6 +508,7 @@ int with_loader(dlbc_gpu_ctx *c, Fn fn) {
     const LoadF64 ld{static_cast<const double *>(c->popA), c->geo.sp, c->q};
     if (c->d == 3) return fn(D3Q19{}, ld);
     if (c->d == 2) return fn(D2Q9{}, ld);
+    if (c->q == 5) return fn(D1Q5{}, ld);
     return fn(D1Q3{}, ld);
   }
   if (c->storage == DLBC_STORAGE_F32_SHIFT) {
@@ -720,15 +722,18 @@ int dlbc_gpu_create(dlbc_gpu_ctx **out, int d, int q, int components, const int6
                     int storage, int device) {
   if (!out || !n) return DLBC_ERR_ARG;
   *out = nullptr;
-  if (!((d == 3 && q == 19) || (d == 2 && q == 9) || (d == 1 && q == 3)))
+  if (!((d == 3 && q == 19) || (d == 2 && q == 9) || (d == 1 && (q == 3 || q == 5))))
     return DLBC_ERR_UNSUPPORTED;
   if (components < 1 || components > kMaxC) return DLBC_ERR_UNSUPPORTED;
   if (storage < DLBC_STORAGE_F64 || storage > DLBC_STORAGE_F16_SHIFT) return DLBC_ERR_ARG;
   if (storage != DLBC_STORAGE_F64 && !(d == 3 && q == 19 && components == 2))
     return DLBC_ERR_UNSUPPORTED;  // shifted storage: the D3Q19 two-component kernel only
   int64_t sites = 1;
+  // the periodic images the reference's halo holds: n >= 2 (halo 2), and
+  // n >= 4 for D1Q5 (halo 4, src/dlbc/lattice.d:96-103)
+  const int64_t nmin = q == 5 ? 4 : 2;
   for (int i = 0; i < d; ++i) {
-    if (n[i] < 2 || n[i] > (1 << 20)) return DLBC_ERR_ARG;
+    if (n[i] < nmin || n[i] > (1 << 20)) return DLBC_ERR_ARG;
     sites *= n[i];
   }
   if (sites > (int64_t)1 << 30) return DLBC_ERR_UNSUPPORTED;
@@ -863,6 +868,9 @@ static int mask_ready(dlbc_gpu_ctx *c, int solid) {
                          c->geo);
     else if (c->d == 2)
       hipLaunchKernelGGL(k_solid_words<D2Q9>, dim3(nb), dim3(256), 0, c->stream, c->mask, c->nbw,
+                         c->geo);
+    else if (c->q == 5)
+      hipLaunchKernelGGL(k_solid_words<D1Q5>, dim3(nb), dim3(256), 0, c->stream, c->mask, c->nbw,
                          c->geo);
     else
       hipLaunchKernelGGL(k_solid_words<D1Q3>, dim3(nb), dim3(256), 0, c->stream, c->mask, c->nbw,

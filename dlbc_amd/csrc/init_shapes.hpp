@@ -34,8 +34,8 @@ struct ShapeSpec {
 
 // eqDistUnity (src/dlbc/lb/eqdist.d:310-315).  For SecondOrder/ThirdOrder the
 // unit equilibrium is the weight vector; for BDist2 it is LB3D's constant set
-// (D3Q19: 1/3, 2/36, 1/36; D2Q9: 1/2, 1/12, 1/24); for D1Q3 BDist2 returns the
-// population [1, 0, 0] itself.
+// (D3Q19: 1/3, 2/36, 1/36; D2Q9: 1/2, 1/12, 1/24); for D1Q3 and D1Q5 BDist2
+// returns the population [1, 0, ...] itself.
 inline bool eq_unity(int d, int q, int form, double *u) {
   if (d == 3 && q == 19) {
     if (form == 2) {
@@ -71,6 +71,17 @@ inline bool eq_unity(int d, int q, int form, double *u) {
     } else {
       u[0] = 4.0 / 6.0;
       u[1] = u[2] = 1.0 / 6.0;
+    }
+    return true;
+  }
+  if (d == 1 && q == 5) {  // BDist2 returns the population, as for D1Q3
+    if (form == 2) {
+      u[0] = 1.0;
+      u[1] = u[2] = u[3] = u[4] = 0.0;
+    } else {
+      u[0] = 6.0 / 12.0;
+      u[1] = u[2] = 2.0 / 12.0;
+      u[3] = u[4] = 1.0 / 12.0;
     }
     return true;
   }

@@ -20,6 +20,7 @@ enum EqForm : int { kSecondOrder = 0, kThirdOrder = 1, kBDist2 = 2 };
 // zero components there.
 struct D3Q19 {
   static constexpr int d = 3, q = 19;
+  static constexpr double css = 1.0 / 3.0;  // c_s^2, connectivity.d:214-248
   // connectivity.d:401-418
   static constexpr int c[19][3] = {
       {0, 0, 0},  {1, 0, 0},   {-1, 0, 0}, {0, 1, 0},   {0, -1, 0}, {0, 0, 1},   {0, 0, -1},
@@ -36,6 +37,7 @@ struct D3Q19 {
 
 struct D2Q9 {
   static constexpr int d = 2, q = 9;
+  static constexpr double css = 1.0 / 3.0;
   // connectivity.d:344-374
   static constexpr int c[9][3] = {{0, 0, 0},  {1, 0, 0},  {-1, 0, 0}, {0, 1, 0},  {0, -1, 0},
                                   {1, 1, 0},  {1, -1, 0}, {-1, 1, 0}, {-1, -1, 0}};
@@ -47,6 +49,7 @@ struct D2Q9 {
 
 struct D1Q3 {
   static constexpr int d = 1, q = 3;
+  static constexpr double css = 1.0 / 3.0;
   // connectivity.d:318-324
   static constexpr int c[3][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}};
   static constexpr int bounce[3] = {0, 2, 1};
@@ -54,7 +57,18 @@ struct D1Q3 {
   static constexpr double w[3] = {4.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0};
 };
 
-constexpr double kCss = 1.0 / 3.0;  // c_s^2 of all three (connectivity.d:214-248)
+// connectivity.d:236-238 (c_s^2 = 1), 279-283, 347-353: the +-2 links reach
+// two sites out, so the reference gives this lattice a halo of 4
+// (lattice.d:96-103); here the periodic wrap serves any length >= 4.
+struct D1Q5 {
+  static constexpr int d = 1, q = 5;
+  static constexpr double css = 1.0;
+  static constexpr int c[5][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {2, 0, 0}, {-2, 0, 0}};
+  static constexpr int bounce[5] = {0, 2, 1, 4, 3};
+  static constexpr double w[5] = {6.0 / 12.0, 2.0 / 12.0, 2.0 / 12.0, 1.0 / 12.0, 1.0 / 12.0};
+};
+
+constexpr double kCss = D3Q19::css;
 
 // density(): std.algorithm.sum = 0.0 + pairwise sum by recursive halving
 // (src/dlbc/lb/density.d:37-39).
@@ -114,7 +128,7 @@ __device__ inline void eq_dist(int form, const double *pop, const double *dv, do
     for (int j = 0; j < D; ++j) vdotv += v[j] * v[j];
     // c_s^2 products exactly as the reference forms them; every quotient
     // by them is the correctly rounded one (div_const)
-    constexpr double css = kCss;
+    constexpr double css = L::css;
     constexpr double css2 = 2.0 * css;
     constexpr double css2p2 = css2 * css;
     constexpr double css6p3 = 3.0 * css2p2 * css;

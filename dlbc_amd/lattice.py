@@ -126,6 +126,13 @@ class Rng:
         return lib().dlbc_rng_uniform(C.byref(self.s), float(a), float(b))
 
 
+def reference_halo(d, q):
+    """The halo of the reference's fields (src/dlbc/lattice.d:96-103): 2, and
+    4 for D1Q5, whose +-2 links and the Shan-Chen stencil on them reach four
+    sites out.  It fixes the order of the random draws of the initialisation."""
+    return 4 if (d == 1 and q == 5) else 2
+
+
 def _padded_shape(n, halo):
     return tuple(int(v) + 2 * halo for v in n)
 
@@ -373,6 +380,7 @@ class Lattice:
         host initialisation and upload, which device_init=False uses)."""
         d, q, nc = p["d"], p["q"], p.get("components", 1)
         n = p["gn"]
+        h = reference_halo(d, q)  # the draw order of the random initialisations
         L = cls(d, q, nc, n, storage=storage, device=device)
         L.set_params(p.get("tau", [1.0] * nc), p.get("gcc"), p.get("gwc"), p.get("globalAcc"),
                      p.get("bodyForce"), p.get("eqDistForm", "SecondOrder"),
@@ -394,15 +402,15 @@ class Lattice:
                              p.get("fluidPerturb", [0.0] * nc)[f], rng, p.get("initRadius", 0.0),
                              p.get("initOffset"), p.get("initSeparation"),
                              p.get("initAxis", "X"), p.get("interfaceThickness", 0.0),
-                             p.get("lamellaeWidths"))
+                             p.get("lamellaeWidths"), halo=h)
                 L.init_eq_dist_wall(f)
             return L
         mask = init_mask(d, n, p.get("maskInit", "None"), p.get("maskAxis", "X"),
                          p.get("wallOffset", 0), p.get("dimensionHasWall"),
-                         p.get("wallThickness", 1))
+                         p.get("wallThickness", 1), halo=h)
         if "maskArray" in p:  # MaskInit.File: physical sites from the file, halo None
             mask[...] = 0
-            mask[tuple(slice(2, 2 + v) for v in n)] = np.asarray(p["maskArray"], np.int32)
+            mask[tuple(slice(h, h + v) for v in n)] = np.asarray(p["maskArray"], np.int32)
         grng = Rng(p.get("seed", 0))  # initRNG on rank 0
         eqf = p.get("eqDistForm", "SecondOrder")
         pops = []
@@ -411,12 +419,12 @@ class Lattice:
             pop = init_fluid(d, q, n, p["fluidInit"][f], p["fluidDensities"][f], eqf,
                              p.get("fluidPerturb", [0.0] * nc)[f], rng, p.get("initRadius", 0.0),
                              p.get("initOffset"), p.get("initSeparation"), p.get("initAxis", "X"),
-                             p.get("interfaceThickness", 0.0), p.get("lamellaeWidths"))
-            init_eq_dist_wall(pop, mask, d, q, n)
+                             p.get("interfaceThickness", 0.0), p.get("lamellaeWidths"), halo=h)
+            init_eq_dist_wall(pop, mask, d, q, n, halo=h)
             pops.append(pop)
-        L.upload_mask(mask, halo=2)
+        L.upload_mask(mask, halo=h)
         for f in range(nc):
-            L.upload_population(f, pops[f], halo=2)
+            L.upload_population(f, pops[f], halo=h)
         return L
 
 

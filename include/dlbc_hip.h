@@ -68,10 +68,12 @@ enum {
 };
 
 /* Create a context for one lattice on HIP device `device`.
- * d, q: connectivity (3,19), (2,9) or (1,3) -- the reference's dub
- * configurations d3q19, d2q9 and d1q3 (src/dlbc/lb/connectivity.d:21-32).
+ * d, q: connectivity (3,19), (2,9), (1,3) or (1,5) -- the reference's
+ * population connectivities d3q19, d2q9, d1q3 and d1q5
+ * (src/dlbc/lb/connectivity.d:21-32).
  * components: number of fluid components C (1..4), lb.components.
- * n: physical lattice lengths n[0..d) (lattice.gn on one rank), each >= 2.
+ * n: physical lattice lengths n[0..d) (lattice.gn on one rank), each >= 2,
+ * and >= 4 for D1Q5, whose reference halo is 4 (src/dlbc/lattice.d:96-103).
  * Replaces the Lattice constructor and field allocation
  * (src/dlbc/lattice.d:166-181, src/dlbc/lb/init.d:60-73, lb/force.d:67-107). */
 int dlbc_gpu_create(dlbc_gpu_ctx **out, int d, int q, int components, const int64_t *n,

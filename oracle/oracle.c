@@ -61,8 +61,9 @@ static const int D3Q19_VEL[19][3] = {
 /* D2Q9 order, connectivity.d:356-374 (D2Q5 prefix 344-354). */
 static const int D2Q9_VEL[9][2] = {{0, 0}, {1, 0}, {-1, 0}, {0, 1}, {0, -1},
                                     {1, 1}, {1, -1}, {-1, 1}, {-1, -1}};
-/* D1Q3 order, connectivity.d:318-324. */
+/* D1Q3 order, connectivity.d:318-324; D1Q5 appends +2 and -2 (347-353). */
 static const int D1Q3_VEL[3][1] = {{0}, {1}, {-1}};
+static const int D1Q5_VEL[5][1] = {{0}, {1}, {-1}, {2}, {-2}};
 
 int or_conn(int d, int q, int *vel, int *bounce, double *w, double *css) {
   int i, j, k;
@@ -86,11 +87,17 @@ int or_conn(int d, int q, int *vel, int *bounce, double *w, double *css) {
     w[0] = 4.0 / 6.0;
     w[1] = 1.0 / 6.0;
     w[2] = 1.0 / 6.0;
+  } else if (d == 1 && q == 5) {
+    for (i = 0; i < 5; i++) vel[i] = D1Q5_VEL[i][0];
+    /* connectivity.d:280-284 */
+    w[0] = 6.0 / 12.0;
+    w[1] = w[2] = 2.0 / 12.0;
+    w[3] = w[4] = 1.0 / 12.0;
   } else {
     return -1;
   }
-  /* c_s^2 = 1/3 for all three, connectivity.d:214-248 */
-  *css = 1.0 / 3.0;
+  /* c_s^2, connectivity.d:214-248: 1 for D1Q5, 1/3 for the others */
+  *css = (d == 1 && q == 5) ? 1.0 : 1.0 / 3.0;
   /* generateBounce, connectivity.d:198-212: first j with c_i + c_j == 0 */
   for (i = 0; i < q; i++) {
     for (j = 0; j < q; j++) {
@@ -123,7 +130,9 @@ or_lattice *or_create(int d, int q, int ncomp, const int *n) {
   L->d = d;
   L->q = q;
   L->nc = ncomp;
-  L->h = 2; /* lattice.d:96-103 (D1Q5 would need 4; not supported) */
+  /* lattice.d:96-103: halo 2, and 4 for D1Q5, whose +-2 links and the
+   * Shan-Chen stencil on them reach four sites out */
+  L->h = (d == 1 && q == 5) ? 4 : 2;
   for (i = 0; i < q; i++) {
     int k;
     for (k = 0; k < 3; k++) L->vel[i][k] = k < d ? vel[i * d + k] : 0;
@@ -555,6 +564,18 @@ void or_advect_fields(or_lattice *L) {
           for (i = 0; i < L->d; i++) /* isOnEdge, advection.d:91-99 */
             if (p[i] == 0 || p[i] == L->nh[i] - 1) edge = 1;
           if (edge) continue;
+          /* D1Q5 on its halo of 4: the second layer's +-2 sources lie outside
+           * the field (the reference indexes past it there); no physical
+           * site ever reads that layer, so it keeps its populations */
+          for (vq = 0; vq < q; vq++)
+            for (i = 0; i < L->d; i++) {
+              const long sp = p[i] - L->vel[vq][i];
+              if (sp < 0 || sp >= L->nh[i]) edge = 1;
+            }
+          if (edge) {
+            memcpy(dst + ip * q, src + ip * q, sizeof(double) * q);
+            continue;
+          }
           if (L->mask[ip] == 0) { /* isAdvectable */
             for (vq = 0; vq < q; vq++) {
               const size_t inb = idx3(L, x - L->vel[vq][0], y - L->vel[vq][1],

@@ -226,8 +226,10 @@ class Oracle:
         lib().or_set_mask(self.h, m)
 
     def raw(self, what, f=0):
-        """Halo-padded storage as a numpy view (fluid, density, psi, force, mask)."""
-        nh = tuple(v + 4 for v in self.n)
+        """Halo-padded storage as a numpy view (fluid, density, psi, force, mask);
+        the halo is 2, and 4 for D1Q5 (src/dlbc/lattice.d:96-103)."""
+        h = 4 if (self.d, self.q) == (1, 5) else 2
+        nh = tuple(v + 2 * h for v in self.n)
         if what == "mask":
             p = lib().or_mask(self.h)
             return np.ctypeslib.as_array(p, shape=nh)

@@ -166,6 +166,46 @@ def test_config2_full_size_1000_steps_bitexact(D, oracle_mod, form):
     _same_state(g, o, "config2 128^3 " + form)
 
 
+def _d1q5_case(form, walls):
+    """D1Q5 (c_s^2 = 1, links of length 2, halo 4 in the reference):
+    lb/laplace-quasi-1d-d1q5.in with a seeded perturbation, optionally with
+    plates, a wall coupling and an acceleration."""
+    p = dict(d=1, q=5, gn=[64], components=2, tau=[1.0, 0.8], eqDistForm=form,
+             gcc=[[0.0, 3.6], [3.6, 0.0]], enableShanChen=True, psiForm="Exponential",
+             fluidInit=["EqDistPerturbFrac"] * 2, fluidDensities=[[0.7], [0.5]],
+             fluidPerturb=[0.2, 0.2], randomReseed=[True, True], randomSeed=[5, 6])
+    if walls:
+        p.update(maskInit="Plates", maskAxis="X", wallOffset=3, gwc=[0.1, -0.2],
+                 globalAcc=[1e-4])
+    return p
+
+
+@pytest.mark.parametrize("form", ["SecondOrder", "ThirdOrder", "BDist2"])
+@pytest.mark.parametrize("walls", [False, True], ids=["periodic", "plates"])
+def test_d1q5_bitexact(D, oracle_mod, form, walls):
+    """D1Q5 through the general two-pass kernels against the oracle, bit for
+    bit, after 300 steps.  (The reference's own D1Q5 test is disabled and
+    holds no data, so the oracle restatement is the pin.)"""
+    p = _d1q5_case(form, walls)
+    g, o = _pair(D, oracle_mod, p)
+    g.step(300)
+    o.run(300)
+    _same_state(g, o, "D1Q5 %s walls=%s" % (form, walls))
+
+
+def test_d1q5_laplace_case_bitexact(D, oracle_mod):
+    """lb/laplace-quasi-1d-d1q5.in as written (ThirdOrder, a centred
+    lamella of 0.7/0.5, 1000 steps)."""
+    p = dict(d=1, q=5, gn=[64], components=2, tau=[1.0, 1.0], eqDistForm="ThirdOrder",
+             gcc=[[0.0, 3.6], [3.6, 0.0]], enableShanChen=True, psiForm="Exponential",
+             fluidInit=["EqDistSphereFrac"] * 2, fluidDensities=[[0.7, 0.5], [0.5, 0.7]],
+             initRadius=0.25, interfaceThickness=5.0)
+    g, o = _pair(D, oracle_mod, p)
+    g.step(1000)
+    o.run(1000)
+    _same_state(g, o, "D1Q5 laplace")
+
+
 def test_walls_wetting_bodyforce_bitexact(D, oracle_mod):
     """Walls with a wall-fluid coupling (gwc) and random solid sites, which
     no reference fixture exercises: GPU == oracle.  (The oracle has no body

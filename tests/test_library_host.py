@@ -91,6 +91,24 @@ def test_other_fluid_inits_equal_oracle(oracle_mod, kind):
     assert np.array_equal(pop.view(np.uint64), o.raw("fluid").view(np.uint64))
 
 
+@pytest.mark.parametrize("kind", ["EqDistPerturb", "ConstRandom", "EqDistSphereFrac",
+                                  "EqDistLamellaeFrac"])
+def test_d1q5_inits_equal_oracle(oracle_mod, kind):
+    """D1Q5 keeps a halo of 4 (src/dlbc/lattice.d:96-103): the random draws
+    run over the 4-site halo, and the shapes see the 4-site offset."""
+    import dlbc_amd as D
+    n = [13]
+    assert D.lattice.reference_halo(1, 5) == 4
+    kw = dict(densities=[0.6, 0.2], perturb=0.05, radius=0.3, widths=[0.25, 0.75])
+    o = oracle_mod.Oracle(1, 5, 1, n)
+    o.set_params([1.0], eq_dist_form=1)
+    o.init_fluid(0, kind, kw["densities"], kw["perturb"], True, 17, kw["radius"], None, None,
+                 0, 2.0, kw["widths"])
+    pop = D.init_fluid(1, 5, n, kind, kw["densities"], "ThirdOrder", kw["perturb"], D.Rng(17),
+                       kw["radius"], None, None, 0, 2.0, kw["widths"], halo=4)
+    assert np.array_equal(pop.view(np.uint64), o.raw("fluid").view(np.uint64))
+
+
 def test_rng_matches_reference_golden_values():
     """random.d:55-89 golden values through the product's Mt19937."""
     import dlbc_amd as D

@@ -15,9 +15,9 @@ import pytest
 def test_connectivity_tables(oracle_mod):
     """lb/connectivity.d:432-480: bounce symmetry, weights sum to one,
     c_s^2 = 1/3, plus the isotropy moments sum_i w_i c_ia c_ib = c_s^2 delta."""
-    for d, q in ((3, 19), (2, 9), (1, 3)):
+    for d, q in ((3, 19), (2, 9), (1, 3), (1, 5)):
         vel, bounce, w, css = oracle_mod.conn(d, q)
-        assert css == 1.0 / 3.0
+        assert css == (1.0 if q == 5 else 1.0 / 3.0)  # connectivity.d:214-248
         assert abs(w.sum() - 1.0) < 1e-15
         for i in range(q):
             assert np.all(vel[bounce[i]] == -vel[i])
@@ -25,6 +25,12 @@ def test_connectivity_tables(oracle_mod):
         m2 = np.einsum("i,ia,ib->ab", w, vel, vel)
         assert np.allclose(m2, css * np.eye(d), atol=1e-15)
         assert np.allclose(np.einsum("i,ia->a", w, vel), 0.0, atol=1e-16)
+    # D1Q5 (connectivity.d:279-283, 347-353): velocities 0, 1, -1, 2, -2 and the
+    # fourth moment sum_i w_i c^4 = 3 c_s^4
+    vel, _, w, _ = oracle_mod.conn(1, 5)
+    assert vel[:, 0].tolist() == [0, 1, -1, 2, -2]
+    assert w.tolist() == [0.5, 1.0 / 6.0, 1.0 / 6.0, 1.0 / 12.0, 1.0 / 12.0]
+    assert abs(np.sum(w * vel[:, 0] ** 4) - 3.0) < 1e-15
     vel, _, _, _ = oracle_mod.conn(3, 19)
     # D3Q19 order fixed by connectivity.d:401-418
     assert vel[7].tolist() == [1, 1, 0] and vel[12].tolist() == [1, 0, -1]
@@ -125,6 +131,32 @@ def test_secondorder_closed_form(oracle_mod):
     assert np.allclose(oracle_mod.eq_dist(3, 19, 0, pop, dv, rho), ref, rtol=1e-14, atol=0)
     ref3 = ref + rho * w * (4.5 * cu ** 3 - 4.5 * (u @ u) * cu)
     assert np.allclose(oracle_mod.eq_dist(3, 19, 1, pop, dv, rho), ref3, rtol=1e-14, atol=0)
+
+
+def test_d1q5_equilibria_closed_form(oracle_mod):
+    """D1Q5 has c_s^2 = 1: n_eq = rho w (1 + c u + (c u)^2 / 2 - u^2 / 2), plus
+    (c u)^3 / 6 - u^2 (c u) / 2 for ThirdOrder; both conserve mass and give
+    the first moment rho (u + dv).  BDist2 has no D1Q5 form and returns the
+    population (eqdist.d:269-271)."""
+    vel, _, w, _ = oracle_mod.conn(1, 5)
+    c = vel[:, 0].astype(float)
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        pop = rng.uniform(0.01, 0.2, 5)
+        dv = rng.uniform(-1e-2, 1e-2, 1)
+        rho = oracle_mod.pop_density(pop)
+        u = (pop @ c) / rho + dv[0]
+        cu = c * u
+        ref = rho * w * (1 + cu + 0.5 * cu ** 2 - 0.5 * u * u)
+        eq2 = oracle_mod.eq_dist(1, 5, 0, pop, dv, rho)
+        assert np.allclose(eq2, ref, rtol=1e-13, atol=0)
+        ref3 = ref + rho * w * (cu ** 3 / 6.0 - 0.5 * u * u * cu)
+        eq3 = oracle_mod.eq_dist(1, 5, 1, pop, dv, rho)
+        assert np.allclose(eq3, ref3, rtol=1e-13, atol=0)
+        for eq in (eq2, eq3):
+            assert abs(eq.sum() - rho) < 1e-14 * rho
+            assert abs(eq @ c - rho * u) < 1e-14
+        assert np.array_equal(oracle_mod.eq_dist(1, 5, 2, pop, dv, rho), pop)
 
 
 def test_psi_known_values(oracle_mod):
