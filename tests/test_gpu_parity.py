@@ -206,6 +206,27 @@ def test_d1q5_laplace_case_bitexact(D, oracle_mod):
     _same_state(g, o, "D1Q5 laplace")
 
 
+def test_quasi_1d_across_dimensions(D):
+    """lb/laplace-quasi-1d/compare-lower-dimensions.py: the same x-lamellae
+    (ThirdOrder, Shan-Chen, 1000 steps) on D3Q19 64x4x4, D2Q9 64x4 and D1Q3 64
+    give density profiles along x equal to 1e-14 (the reference's bound)."""
+    base = dict(components=2, tau=[1.0, 1.0], eqDistForm="ThirdOrder",
+                gcc=[[0.0, 3.6], [3.6, 0.0]], enableShanChen=True, psiForm="Exponential",
+                fluidInit=["EqDistLamellaeFrac"] * 2,
+                fluidDensities=[[0.5, 0.7, 0.5], [0.7, 0.5, 0.7]],
+                lamellaeWidths=[0.25, 0.50, 0.25], interfaceThickness=5.0)
+    prof = {}
+    for d, q, gn in ((3, 19, [64, 4, 4]), (2, 9, [64, 4]), (1, 3, [64])):
+        L = D.Lattice.from_params(dict(base, d=d, q=q, gn=gn))
+        L.step(1000)
+        prof[d] = [L.density(f).reshape(64, -1) for f in range(2)]
+    for f in range(2):
+        for d in (3, 2):
+            # uniform across y (and z), and equal to the D1Q3 profile
+            assert np.abs(prof[d][f] - prof[d][f][:, :1]).max() <= 1e-14
+            assert np.abs(prof[d][f][:, 0] - prof[1][f][:, 0]).max() <= 1e-14
+
+
 def test_walls_wetting_bodyforce_bitexact(D, oracle_mod):
     """Walls with a wall-fluid coupling (gwc) and random solid sites, which
     no reference fixture exercises: GPU == oracle.  (The oracle has no body

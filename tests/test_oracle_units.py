@@ -308,3 +308,25 @@ def test_shan_chen_wall_force_closed_form(oracle_mod):
         assert np.allclose(F[f][:, :, -2, 2], -fz, rtol=4e-16, atol=0)
         assert np.all(np.abs(F[f][:, :, 1:-1, :2]) <= 1e-18)
         assert np.all(F[f][:, :, 2:-2, :] == 0.0)
+
+
+def test_quasi_1d_across_dimensions(oracle_mod):
+    """lb/laplace-quasi-1d/compare-lower-dimensions.py on the oracle: the same
+    x-lamellae (ThirdOrder, Shan-Chen, 1000 steps) on D3Q19 64x4x4, D2Q9 64x4
+    and D1Q3 64 give density profiles equal to 1e-14.  A wrong weight, speed
+    of sound or stencil term in one connectivity breaks this."""
+    base = dict(components=2, tau=[1.0, 1.0], eqDistForm="ThirdOrder",
+                gcc=[[0.0, 3.6], [3.6, 0.0]], enableShanChen=True, psiForm="Exponential",
+                fluidInit=["EqDistLamellaeFrac"] * 2,
+                fluidDensities=[[0.5, 0.7, 0.5], [0.7, 0.5, 0.7]],
+                lamellaeWidths=[0.25, 0.50, 0.25], interfaceThickness=5.0)
+    prof = {}
+    for d, q, gn in ((3, 19, [64, 4, 4]), (2, 9, [64, 4]), (1, 3, [64])):
+        o = oracle_mod.Oracle.from_params(dict(base, d=d, q=q, gn=gn))
+        o.run(1000)
+        prof[d] = [o.density(f).reshape(64, -1) for f in range(2)]
+    for f in range(2):
+        for d in (3, 2):
+            assert np.abs(prof[d][f] - prof[d][f][:, :1]).max() == 0.0
+            assert np.abs(prof[d][f][:, 0] - prof[1][f][:, 0]).max() <= 1e-14
+
