@@ -135,6 +135,24 @@ def export_planes(name, test, fields, steps, tol=None, extra=None):
     save(name, arrays, meta)
 
 
+def export_lb3d(name, test, files, steps, nspatial, tol=None):
+    """Export the LB3D (Fortran code) outputs a test keeps beside its own
+    reference data (``reference-data-lb3d``): ``files`` maps a field name to
+    the LB3D file prefix."""
+    d = os.path.join(REF, test, "reference-data-lb3d")
+    arrays, shapes, reduced = {}, {}, {}
+    for fld, prefix in files.items():
+        for t in steps:
+            a = read_outarray(find(d, r"%s_t%08d-\d+\.h5" % (re.escape(prefix), t)))
+            key = "%s_t%d" % (fld.replace("-", "_"), t)
+            shapes[key] = list(a.shape)
+            a, red = reduce_invariant(a, nspatial)
+            reduced[key] = red
+            arrays[key] = a
+    save(name, arrays, {"source": "tests/runnable/tests/%s/reference-data-lb3d" % test,
+                        "shapes": shapes, "reduced_axes": reduced, "tolerance": tol})
+
+
 def main():
     if not os.path.isdir(REF):
         sys.exit("reference tree not present")
@@ -153,6 +171,11 @@ def main():
                  "colour-red-blue"],
                 [0, 10, 100], 3, tol={"t0": 0.0, "t10": 1e-15, "t100": 1e-15},
                 extra={"params": "lb/bdist2-sc-3d/bdist2-sc-3d.in"})
+    # The same run by LB3D (compare-lb3d.sh: h5diff -d 1e-15 against the above).
+    export_lb3d("bdist2_sc_3d_lb3d", "lb/bdist2-sc-3d",
+                {"population-red": "od_pop_dlbc-sc", "density-red": "od_dlbc-sc",
+                 "population-blue": "wd_pop_dlbc-sc", "density-blue": "wd_dlbc-sc"},
+                [0, 100], 3, tol={"t0": 1e-15, "t100": 1e-15})
     export_full("bdist2_sc_2d", "lb/bdist2-sc-2d",
                 ["population-red", "population-blue", "density-red", "density-blue",
                  "colour-red-blue"],

@@ -340,13 +340,17 @@ def test_merging_droplets_3d_against_reference_t1000(D):
 
 
 def test_bdist2_sc_3d_full_size_against_reference(D):
-    """lb/bdist2-sc-3d at its real 64^3 size for 100 steps."""
+    """lb/bdist2-sc-3d at its real 64^3 size for 100 steps, against the
+    reference's data and against LB3D's run of the same case."""
     gd, meta = golden("bdist2_sc_3d")
+    lb3d, meta3 = golden("bdist2_sc_3d_lb3d")
     g = D.Lattice.from_params(case("bdist2_sc_3d"))
     g.step(100)
     for f, c in enumerate(("red", "blue")):
         key = "population_%s_t100" % c
-        assert_close(g.population(f), expand(gd[key], meta["shapes"][key]), 1e-15, key)
+        pop = g.population(f)
+        assert_close(pop, expand(gd[key], meta["shapes"][key]), 1e-15, key)
+        assert_close(pop, expand(lb3d[key], meta3["shapes"][key]), 1e-15, "LB3D " + key)
 
 
 def test_errors_are_returned_not_fatal(D):
