@@ -7,7 +7,7 @@
  *
  * Layout follows the reference: every field is stored array-of-structures,
  * row-major over the halo-padded local lattice [n0+2h][n1+2h][n2+2h][k]
- * (x slowest, component index innermost), halo h = 2
+ * (x slowest, component index innermost), halo h = 2, and 4 for D1Q5
  * (src/dlbc/lattice.d:96-103, src/dlbc/fields/field.d:35-215).
  * One MPI rank is modelled; the halo exchange is the periodic self-exchange
  * of src/dlbc/fields/parallel.d:40-153.
@@ -16,8 +16,9 @@
  * reference's D std.math.exp.  or_exp is correctly rounded (checked against
  * __float128 expq and mpmath); the reference's golden data pin psi and the
  * trajectories built on it only to their 1e-15 .. 1e-14 tolerances.
- * SecondOrder + Shan-Chen trajectories and C > 2 have no reference golden
- * data: their building blocks are pinned one by one (tests/test_oracle_*.py).
+ * SecondOrder + Shan-Chen trajectories, C > 2 and D1Q5 trajectories have no
+ * reference golden data: their building blocks are pinned one by one
+ * (tests/test_oracle_*.py).
  */
 #ifndef DLBC_ORACLE_H
 #define DLBC_ORACLE_H
