@@ -60,6 +60,10 @@ struct dlbc_gpu_ctx {
   // scratch for reductions and transfers
   double *scratch = nullptr;
   size_t scratch_bytes = 0;
+  // dlbc_gpu_step_graph: two captured steps per buffer parity, and the host
+  // state their kernel arguments were captured from
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  std::vector<unsigned char> graph_key;
 };
 
 namespace {
@@ -367,7 +371,9 @@ struct Timer {
 // this lattice's own work list (8 x 32 tiles, x cut into chunks until the
 // launch holds at least target_blocks workgroups, 512 by default: FP32
 // arithmetic leaves room for two 512-thread workgroups per CU).
-int launch_shifted_self(dlbc_gpu_ctx *c) {
+// The work list and the device table of that launch, brought up to date
+// (copies only when the tile target, a parameter or a pointer changed).
+int shifted_self_ready(dlbc_gpu_ctx *c) {
   const int target = c->target_blocks > 0 ? c->target_blocks : 512;
   if (c->items_target != target) {
     const int tiles = ((c->geo.ny + 7) / 8) * ((c->geo.nz + 31) / 32);
@@ -399,6 +405,12 @@ int launch_shifted_self(dlbc_gpu_ctx *c) {
     c->self_tab = t;
     c->self_valid = true;
   }
+  return DLBC_OK;
+}
+
+int launch_shifted_self(dlbc_gpu_ctx *c) {
+  const int rc0 = shifted_self_ready(c);
+  if (rc0) return rc0;
   const int parity = c->popA == c->bufX ? 0 : 1;
   const int grid = (c->nitems + 7) / 8 * 8;
   const int rc = c->storage == DLBC_STORAGE_F32_SHIFT
@@ -551,22 +563,53 @@ int prepare(dlbc_gpu_ctx *c) {
   return DLBC_OK;
 }
 
+// One step under schedule `sched`: kernel launches on the library stream and
+// the swap of the two population buffers, nothing else (so it can be captured).
+int step_once(dlbc_gpu_ctx *c, int sched, Timer *tm) {
+  tm->slot = 0;
+  if (sched == DLBC_SCHED_FUSED) return dispatch<FusedStep>(c, tm, (double *)nullptr);
+  if (sched == DLBC_SCHED_SPLIT) return dispatch<SplitStep>(c, tm, (double *)nullptr);
+  return dispatch<TwoPassStep>(c, tm, (double *)nullptr);
+}
+
 int do_steps(dlbc_gpu_ctx *c, int nsteps, Timer *tm) {
   int rc = prepare(c);
   if (rc) return rc;
   const int sched = effective_schedule(c);
   for (int t = 0; t < nsteps; ++t) {
-    tm->slot = 0;
-    if (sched == DLBC_SCHED_FUSED)
-      rc = dispatch<FusedStep>(c, tm, (double *)nullptr);
-    else if (sched == DLBC_SCHED_SPLIT)
-      rc = dispatch<SplitStep>(c, tm, (double *)nullptr);
-    else
-      rc = dispatch<TwoPassStep>(c, tm, (double *)nullptr);
+    rc = step_once(c, sched, tm);
     if (rc) return rc;
   }
   HIPCHK(c, hipGetLastError());
   return DLBC_OK;
+}
+
+void drop_graphs(dlbc_gpu_ctx *c) {
+  for (auto &g : c->graph) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
+}
+
+// Everything on the host that the kernel arguments of a step are taken from:
+// schedule, launch shape, geometry (with the block order), parameters and the
+// device table and work list of the shifted tiled kernel.  Buffers never move
+// after creation; the two population buffers swap every step, which the
+// parity index of the graphs covers.
+std::vector<unsigned char> graph_key(const dlbc_gpu_ctx *c, int sched) {
+  std::vector<unsigned char> k;
+  auto put = [&](const void *p, size_t n) {
+    const auto *b = static_cast<const unsigned char *>(p);
+    k.insert(k.end(), b, b + n);
+  };
+  const int ints[4] = {sched, c->storage, c->has_solid, c->target_blocks};
+  put(ints, sizeof ints);
+  put(&c->geo, sizeof c->geo);
+  put(&c->p, sizeof c->p);
+  put(&c->sparams, sizeof c->sparams);
+  put(&c->d_items, sizeof c->d_items);
+  put(&c->nitems, sizeof c->nitems);
+  return k;
 }
 
 // host index of physical site (x,y,z) in a halo-padded array of the lattice
@@ -1112,6 +1155,60 @@ int dlbc_gpu_step(dlbc_gpu_ctx *c, int nsteps) {
   return do_steps(c, nsteps, &tm);
 }
 
+// Steps per captured graph (even).
+constexpr int kGraphSteps = 32;
+
+int dlbc_gpu_step_graph(dlbc_gpu_ctx *c, int nsteps) {
+  if (!c || nsteps < 0) return DLBC_ERR_ARG;
+  if (nsteps == 0) return DLBC_OK;
+  int rc = prepare(c);
+  if (rc) return rc;
+  const int sched = effective_schedule(c);
+  if (c->storage != DLBC_STORAGE_F64 && sched == DLBC_SCHED_FUSED) {
+    // the tiled kernel reads its parameters from the device table: refresh it
+    // before every replay, not only before a capture
+    rc = shifted_self_ready(c);
+    if (rc) return fail(c, rc, "shifted tiled kernel: cannot prepare its table");
+  }
+  auto key = graph_key(c, sched);
+  if (key != c->graph_key) {
+    drop_graphs(c);
+    c->graph_key.swap(key);
+  }
+  Timer tm{c, false};
+  int t = 0;
+  if (nsteps >= kGraphSteps) {
+    const int par = c->popA == c->bufX ? 0 : 1;
+    if (!c->graph[par]) {  // an even count: it ends on the buffers it started from
+      void *const a = c->popA, *const b = c->popB;
+      hipGraph_t g = nullptr;
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int rcc = 0;
+      for (int k = 0; k < kGraphSteps && !rcc; ++k) rcc = step_once(c, sched, &tm);
+      const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+      c->popA = a;  // captured, not run
+      c->popB = b;
+      if (rcc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rcc;
+      }
+      HIPCHK(c, ec);
+      const hipError_t ei = hipGraphInstantiate(&c->graph[par], g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ei != hipSuccess) c->graph[par] = nullptr;
+      HIPCHK(c, ei);
+    }
+    for (; t + kGraphSteps <= nsteps; t += kGraphSteps)
+      HIPCHK(c, hipGraphLaunch(c->graph[par], c->stream));
+  }
+  for (; t < nsteps; ++t) {
+    rc = step_once(c, sched, &tm);
+    if (rc) return rc;
+  }
+  HIPCHK(c, hipGetLastError());
+  return DLBC_OK;
+}
+
 int dlbc_gpu_advect(dlbc_gpu_ctx *c) {
   if (!c) return DLBC_ERR_ARG;
   int rc = prepare(c);
@@ -1328,6 +1425,7 @@ int64_t dlbc_gpu_device_bytes(const dlbc_gpu_ctx *c) { return c ? c->dev_bytes :
 void dlbc_gpu_destroy(dlbc_gpu_ctx *c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_graphs(c);
   if (c->popA) (void)hipFree(c->popA);
   if (c->popB) (void)hipFree(c->popB);
   if (c->S) (void)hipFree(c->S);

@@ -55,6 +55,7 @@ def lib():
         "dlbc_gpu_upload_populations": (i, [vp, i, _dp, i]),
         "dlbc_gpu_upload_mask": (i, [vp, _ip, i]),
         "dlbc_gpu_step": (i, [vp, i]),
+        "dlbc_gpu_step_graph": (i, [vp, i]),
         "dlbc_gpu_advect": (i, [vp]),
         "dlbc_gpu_step_timed": (i, [vp, i, C.POINTER(d), opt_dp, C.POINTER(i)]),
         "dlbc_gpu_kernel_name": (C.c_char_p, [vp, i]),
@@ -284,8 +285,11 @@ class Lattice:
         """initEqDistWall (src/dlbc/fields/init.d:469-477) on the device."""
         self._check(lib().dlbc_gpu_init_eq_dist_wall(self.h, int(f)), "init_eq_dist_wall")
 
-    def step(self, nsteps=1):
-        self._check(lib().dlbc_gpu_step(self.h, int(nsteps)), "step")
+    def step(self, nsteps=1, graph=False):
+        """nsteps time steps (src/main.d:174-200); graph=True replays them
+        from a captured two-step HIP graph (same bits, less launch latency)."""
+        fn = lib().dlbc_gpu_step_graph if graph else lib().dlbc_gpu_step
+        self._check(fn(self.h, int(nsteps)), "step")
 
     def advect(self):
         """advectFields alone (streaming with bounce-back)."""

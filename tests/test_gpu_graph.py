@@ -1,0 +1,140 @@
+"""Graph replay of single-lattice steps (dlbc_gpu_step_graph).
+
+The captured steps (32 per graph) are replayed several times with eager work between
+the replays (field downloads, eager steps, odd step counts that flip the
+buffer parity, parameter, schedule, launch and block-order changes that must
+trigger a new capture), and every stage is compared bit for bit with the same
+sequence stepped eagerly (dlbc_gpu_step, itself bit-identical to the oracle).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def D():
+    import dlbc_amd
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("no GPU visible")
+    return dlbc_amd
+
+
+def _d2q9(n=64):
+    return dict(d=2, q=9, gn=[n, n], components=1, tau=[1.0], fluidInit=["EqDistPerturb"],
+                fluidDensities=[[1.0]], fluidPerturb=[1e-3], randomReseed=[True],
+                randomSeed=[42])
+
+
+def _spinodal(n, form="BDist2"):
+    return dict(d=3, q=19, gn=list(n), components=2, tau=[1.0, 1.0], eqDistForm=form,
+                gcc=[[0.0, 3.6], [3.6, 0.0]], enableShanChen=True, psiForm="Exponential",
+                fluidInit=["EqDistPerturb"] * 2, fluidDensities=[[0.7], [0.7]],
+                fluidPerturb=[0.01, 0.01], randomReseed=[True, True], randomSeed=[42, 43])
+
+
+def _walls(n, form="BDist2"):
+    rng = np.random.default_rng(7)
+    mask = (rng.random(n) < 0.12).astype(np.int32)
+    mask[:, :, 0] = 1
+    mask[:, :, -1] = 1
+    return dict(_spinodal(n, form), tau=[0.9, 1.1], gwc=[0.2, -0.1],
+                globalAcc=[1e-5, 0.0, 0.0], fluidDensities=[[0.7], [0.3]],
+                fluidPerturb=[0.05, 0.05], maskArray=mask)
+
+
+# (id, parameters, storage, schedule)
+CASES = [
+    ("d2q9-bgk", _d2q9(), "f64", None),
+    ("d3q19-split", _spinodal((16, 20, 24)), "f64", None),
+    ("d3q19-split-zrows", _spinodal((6, 7, 256)), "f64", None),
+    ("d3q19-fused-walls", _walls((16, 16, 32)), "f64", "fused"),
+    ("d3q19-twopass-walls", _walls((12, 10, 14), "SecondOrder"), "f64", "two_pass"),
+    ("f32-split-walls", _walls((16, 20, 24)), "f32_shift", None),
+    ("f16-fused", _spinodal((20, 16, 40)), "f16_shift", "fused"),
+]
+
+
+def _same(a, b, what):
+    for f in range(a.nc):
+        pa, pb = a.population(f), b.population(f)
+        assert np.array_equal(pa.view(np.uint64), pb.view(np.uint64)), (what, f)
+
+
+@pytest.mark.parametrize("name,p,storage,schedule", CASES, ids=[c[0] for c in CASES])
+def test_graph_replays_equal_eager_steps(D, name, p, storage, schedule):
+    a = D.Lattice.from_params(p, storage=storage, schedule=schedule)
+    b = D.Lattice.from_params(p, storage=storage, schedule=schedule)
+    # 32 steps per graph: counts above, below and between multiples of it;
+    # odd counts flip the parity
+    for k, n in enumerate([40, 71, 2, 33, 64, 5]):
+        a.step(n, graph=True)
+        b.step(n)
+        _same(a, b, "replay %d (%d steps)" % (k, n))
+        for L in (a, b):  # eager work between the replays
+            L.density(0)
+    for L in (a, b):  # new parameters: the graphs are captured again
+        if L.nc == 1:
+            L.set_params([0.8])
+        else:
+            L.set_params([1.2, 0.95], [[0.0, 4.0], [4.0, 0.0]],
+                         [0.1, -0.1] if "walls" in name else None, None, None, "BDist2",
+                         "Exponential", True)
+    a.step(66, graph=True)
+    b.step(66)
+    _same(a, b, "after set_params")
+    if storage == "f64" and a.nc == 2:
+        for L in (a, b):  # the launch shape and block order are baked in too
+            L.set_launch(96)
+            L.set_block_order(3)
+        a.step(34, graph=True)
+        b.step(34)
+        _same(a, b, "after set_launch / set_block_order")
+    a.step(3)  # eager steps on the graph's lattice, then replays again
+    b.step(3)
+    a.step(40, graph=True)
+    b.step(40)
+    _same(a, b, "eager then replay")
+
+
+def test_graph_schedule_switch(D):
+    """The same lattice replayed under one schedule, switched to another,
+    replayed again: the stale graphs are not reused."""
+    p = _walls((16, 16, 32))  # FP64 fused kernel: n_y % 8 == 0, n_z % 32 == 0
+    a = D.Lattice.from_params(p)
+    b = D.Lattice.from_params(p)
+    for s in ["split", "fused", "two_pass", "split"]:
+        a.set_schedule(s)
+        b.set_schedule(s)
+        a.step(34, graph=True)
+        b.step(34)
+        _same(a, b, s)
+
+
+def test_graph_config1_d2q9_1000_steps_bitexact(D, oracle_mod):
+    """D2Q9 single-component BGK 64x64, 1000 steps replayed from the graph,
+    against the oracle (populations, densities and velocities bitwise)."""
+    p = _d2q9()
+    g = D.Lattice.from_params(p)
+    o = oracle_mod.Oracle.from_params(p)
+    g.step(1000, graph=True)
+    o.run(1000)
+    assert np.array_equal(g.population(0).view(np.uint64), o.population(0).view(np.uint64))
+    assert np.array_equal(g.density(0).view(np.uint64), o.density(0).view(np.uint64))
+    assert np.array_equal(g.velocity(0).view(np.uint64), o.velocity(0).view(np.uint64))
+
+
+def test_graph_after_mask_change(D):
+    """A new mask (solid densities recomputed before the replay) under the
+    same graphs."""
+    p = _walls((16, 20, 24))
+    a = D.Lattice.from_params(p)
+    b = D.Lattice.from_params(p)
+    a.step(34, graph=True)
+    b.step(34)
+    for L in (a, b):
+        L.init_mask("Walls", dim_has_wall=[[False, False], [True, True], [False, False]])
+    a.step(36, graph=True)
+    b.step(36)
+    _same(a, b, "after init_mask")
