@@ -40,6 +40,23 @@ struct BatchLattice {
   ShiftParams p;
 };
 
+// The table as the kernels read it: in the constant address space.  A
+// pointer loaded from generic memory is generic to the compiler, and every
+// access through it becomes a flat instruction, which also waits on the LDS
+// counter; a pointer loaded from constant memory is taken for global memory
+// (global loads and stores with a scalar base).  The host writes the table
+// before the launches that read it, never during one.  (The host pass only
+// parses the kernels; there the qualifier is left out.)
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(4))) const BatchLattice ConstLattice;
+#else
+typedef const BatchLattice ConstLattice;
+#endif
+
+__device__ inline ConstLattice &table_entry(const BatchLattice *lats, int l) {
+  return ((ConstLattice *)lats)[l];
+}
+
 // A work item: lattice index, tile origin (y0, z0), planes [xs, xe).
 struct BatchItem {
   int lat, y0, z0, xs, xe, pad0, pad1, pad2;
@@ -63,7 +80,7 @@ __device__ inline void ring_coord_v(int r, int vy, int vz, int &ey, int &ez) {
 // (ring size and coordinates known at compile time); SOLID: the lattice has
 // solid sites.
 template <class Store, int TY, bool FULL, bool SOLID>
-__device__ __forceinline__ void batch_tile(const BatchItem &item, const BatchLattice &B,
+__device__ __forceinline__ void batch_tile(const BatchItem &item, ConstLattice &B,
                                            int parity, float (*psi_lds)[2][FusedShape<TY>::EY *
                                                                            FusedShape<TY>::EZ],
                                            float (*stage)[FusedShape<TY>::ring_items]) {
@@ -81,7 +98,7 @@ __device__ __forceinline__ void batch_tile(const BatchItem &item, const BatchLat
     const Geo g = B.g;
     // Parameters: uniform ones read from the table, per-component ones selected
     // into registers once per item (a lane holds component f).
-    const ShiftParams &p = B.p;
+    const ShiftParams p = B.p;
     const ShiftLane pl = shift_lane(p, f);
     const float rho0_0 = p.rho0[0], rho0_1 = p.rho0[1];
     const int psi_form = p.psi_form;
@@ -261,7 +278,7 @@ __global__ void __launch_bounds__(64 * TY, SOLID ? 4 : 1)
   const int it0 = xcd * seg + j, it1 = min(nitems, (xcd + 1) * seg);
   for (int it = it0; it < it1; it += per_xcd) {
     const BatchItem item = items[it];
-    const BatchLattice &B = lats[item.lat];
+    ConstLattice &B = table_entry(lats, item.lat);
     const bool full = item.y0 + TY <= B.g.ny && item.z0 + Sh::TZ <= B.g.nz;
     if (full)
       batch_tile<Store, TY, true, SOLID>(item, B, parity, psi_lds, stage);
@@ -340,12 +357,14 @@ __global__ void __launch_bounds__(kShiftThreads)
   const int b = blk0 + blockIdx.x;
   if (b >= blk1) return;
   const int l = batch_block_lattice(first, nlat, b);
-  const BatchLattice &B = lats[l];
+  ConstLattice &B = table_entry(lats, l);
   const Store *A = static_cast<const Store *>(B.pop[parity]);
-  if (B.p.has_solid)
-    ssplit_psi_block<Store, true, false>(A, B.nbw, B.P, B.g, B.p, b - first[l]);
+  const Geo g = B.g;
+  const ShiftParams p = B.p;
+  if (p.has_solid)
+    ssplit_psi_block<Store, true, false>(A, B.nbw, B.P, g, p, b - first[l]);
   else
-    ssplit_psi_block<Store, false, false>(A, B.nbw, B.P, B.g, B.p, b - first[l]);
+    ssplit_psi_block<Store, false, false>(A, B.nbw, B.P, g, p, b - first[l]);
 }
 
 template <class Store>
@@ -356,15 +375,17 @@ __global__ void __launch_bounds__(kShiftThreads)
   const int b = blk0 + blockIdx.x;
   if (b >= blk1) return;  // whole block: no barrier is left waiting
   const int l = batch_block_lattice(first, nlat, b);
-  const BatchLattice &B = lats[l];
+  ConstLattice &B = table_entry(lats, l);
   const Store *A = static_cast<const Store *>(B.pop[parity]);
   Store *W = static_cast<Store *>(const_cast<void *>(B.pop[parity ^ 1]));
-  if (B.p.has_solid)
-    ssplit_collide_block<Store, true, false>(A, W, B.nbw, B.S, B.P, B.g, B.p, b - first[l],
-                                             xch, nullptr);
+  const Geo g = B.g;
+  const ShiftParams p = B.p;
+  if (p.has_solid)
+    ssplit_collide_block<Store, true, false>(A, W, B.nbw, B.S, B.P, g, p, b - first[l], xch,
+                                             nullptr);
   else
-    ssplit_collide_block<Store, false, false>(A, W, B.nbw, B.S, B.P, B.g, B.p, b - first[l],
-                                              xch, nullptr);
+    ssplit_collide_block<Store, false, false>(A, W, B.nbw, B.S, B.P, g, p, b - first[l], xch,
+                                              nullptr);
 }
 
 // One step of the blocks [blk0, blk0 + nblk) of a batch on the split schedule.

@@ -91,21 +91,46 @@ __device__ inline const T &at(const T *base, int64_t k) {
   return *elem<OFF32>(const_cast<T *>(base), k);
 }
 
+// Index offsets from a site to its neighbours along each axis with the
+// periodic wrap folded in (m: -1, p: +1), set up once per lane: a neighbour
+// index is then the site plus up to three of them, instead of three wraps
+// (compare and select each) and two 32-bit multiplies per neighbour.
+struct NbrOffsets {
+  int xm, xp, ym, yp, zm, zp;
+};
+
+__device__ inline NbrOffsets nbr_offsets(const Geo &g, int x, int y, int z) {
+  const int plane = g.ny * g.nz;
+  NbrOffsets o;
+  o.xm = x == 0 ? (g.nx - 1) * plane : -plane;
+  o.xp = x == g.nx - 1 ? -(g.nx - 1) * plane : plane;
+  o.ym = y == 0 ? (g.ny - 1) * g.nz : -g.nz;
+  o.yp = y == g.ny - 1 ? -(g.ny - 1) * g.nz : g.nz;
+  o.zm = z == 0 ? g.nz - 1 : -1;
+  o.zp = z == g.nz - 1 ? -(g.nz - 1) : 1;
+  return o;
+}
+
+// Site s + (cx, cy, cz), each component in {-1, 0, 1} (the same index as
+// site_of over the wrapped coordinates).
+__device__ inline int nbr_site(int s, const NbrOffsets &o, int cx, int cy, int cz) {
+  return s + (cx < 0 ? o.xm : (cx > 0 ? o.xp : 0)) + (cy < 0 ? o.ym : (cy > 0 ? o.yp : 0)) +
+         (cz < 0 ? o.zm : (cz > 0 ? o.zp : 0));
+}
+
 // True when every population index of a lattice (19 sp) is below 2^29.
 inline bool split_off32(const Geo &g) { return 19 * g.sp < ((int64_t)1 << 29); }
 
 // The 19 streamed populations of component f (array Af) at site s = (x, y, z)
 // whose solid word is sw (pull with bounce-back, src/dlbc/lb/advection.d:61-89).
 template <bool SOLID, bool OFF32>
-__device__ inline void split_pull(const double *__restrict__ Af, const Geo &g, int s, int x,
-                                  int y, int z, uint32_t sw, double *n) {
+__device__ inline void split_pull(const double *__restrict__ Af, const Geo &g, int s,
+                                  const NbrOffsets &o, uint32_t sw, double *n) {
   using L = D3Q19;
 #pragma unroll
   for (int i = 0; i < L::q; ++i) {
     const bool bb = SOLID && ((sw >> L::bounce[i]) & 1u);
-    const int src = bb ? s
-                       : site_of(wrap(x - L::c[i][0], g.nx), wrap(y - L::c[i][1], g.ny),
-                                 wrap(z - L::c[i][2], g.nz), g);
+    const int src = bb ? s : nbr_site(s, o, -L::c[i][0], -L::c[i][1], -L::c[i][2]);
     const int comp = bb ? L::bounce[i] : i;
     n[i] = at<OFF32>(Af, (int64_t)comp * g.sp + src);
   }
@@ -164,7 +189,7 @@ __global__ void __launch_bounds__(kSplitThreads)
   int x, y, z;
   coords(s, g, x, y, z);
   double n[19];
-  split_pull<SOLID, OFF32>(A + (int64_t)f * 19 * g.sp, g, s, x, y, z, sw, n);
+  split_pull<SOLID, OFF32>(A + (int64_t)f * 19 * g.sp, g, s, nbr_offsets(g, x, y, z), sw, n);
   *elem<OFF32>(S, (int64_t)f * g.sp + s) = psi_of(psi_form, density<D3Q19>(n));
 }
 
@@ -318,13 +343,15 @@ __global__ void __launch_bounds__(kSplitThreads)
   const bool inside = split_lane(g, s, f);
   const int k = ((threadIdx.x >> 7) << 6) | (threadIdx.x & 63);  // site within the block
   const uint32_t sw = !inside ? (1u << 31) : (SOLID ? nbw[s] : 0u);
-  int x = 0, y = 0, z = 0;
+  NbrOffsets o{};
   double n[19];
   double rho = 0.0, mom[3] = {0.0, 0.0, 0.0};
   const bool bdist2 = p.eq_form == kBDist2;
   if (!(sw >> 31)) {
+    int x, y, z;
     coords(s, g, x, y, z);
-    split_pull<SOLID, OFF32>(A + (int64_t)f * 19 * g.sp, g, s, x, y, z, sw, n);
+    o = nbr_offsets(g, x, y, z);
+    split_pull<SOLID, OFF32>(A + (int64_t)f * 19 * g.sp, g, s, o, sw, n);
   }
   if (staged) stage_psi_rows<SB, kSplitThreads, OFF32>(S, g, split_block(g) * SB, rows);
   if (!(sw >> 31)) {
@@ -344,10 +371,7 @@ __global__ void __launch_bounds__(kSplitThreads)
 #pragma unroll
     for (int d = 0; d < 3; ++d) mom_p[d] = xch[f ^ 1][1 + d][k];
   }
-  auto nbr = [&](int i) {
-    return site_of(wrap(x + L::c[i][0], g.nx), wrap(y + L::c[i][1], g.ny),
-                   wrap(z + L::c[i][2], g.nz), g);
-  };
+  auto nbr = [&](int i) { return nbr_site(s, o, L::c[i][0], L::c[i][1], L::c[i][2]); };
   double F[3];
   auto psi = [&](int ff, int i) {
     if (staged) return staged_psi<SB>(rows, ff, i, k);
@@ -380,15 +404,13 @@ __global__ void __launch_bounds__(kSplitThreads)
 // S still gives the density at solid neighbours (wall force).
 
 template <class Store, bool SOLID, bool OFF32>
-__device__ inline void ssplit_pull(const Store *__restrict__ Af, const Geo &g, int s, int x,
-                                   int y, int z, uint32_t sw, float *n) {
+__device__ inline void ssplit_pull(const Store *__restrict__ Af, const Geo &g, int s,
+                                   const NbrOffsets &o, uint32_t sw, float *n) {
   using L = D3Q19;
 #pragma unroll
   for (int i = 0; i < L::q; ++i) {
     const bool bb = SOLID && ((sw >> L::bounce[i]) & 1u);
-    const int src = bb ? s
-                       : site_of(wrap(x - L::c[i][0], g.nx), wrap(y - L::c[i][1], g.ny),
-                                 wrap(z - L::c[i][2], g.nz), g);
+    const int src = bb ? s : nbr_site(s, o, -L::c[i][0], -L::c[i][1], -L::c[i][2]);
     const int comp = bb ? L::bounce[i] : i;
     n[i] = to_f(at<OFF32>(Af, (int64_t)comp * g.sp + src));
   }
@@ -409,15 +431,18 @@ __device__ inline void ssplit_psi_block(const Store *__restrict__ A,
   int x, y, z;
   coords(s, g, x, y, z);
   float n[19];
-  ssplit_pull<Store, SOLID, OFF32>(A + (int64_t)f * 19 * g.sp, g, s, x, y, z, sw, n);
+  ssplit_pull<Store, SOLID, OFF32>(A + (int64_t)f * 19 * g.sp, g, s, nbr_offsets(g, x, y, z), sw,
+                                   n);
   const float rho0 = f ? p.rho0[1] : p.rho0[0];
   *elem<OFF32>(P, (int64_t)f * g.sp + s) = psi_f(p.psi_form, rho0 + shifted_drho(n));
 }
 
 // Pass 2 for block blk of a lattice; xch: LDS for the partner exchange;
-// rows: LDS for the block's psi rows (used when n_z is a multiple of the
-// block's sites), or null.
-template <class Store, bool SOLID, bool OFF32>
+// rows: LDS for the block's psi rows when STAGED (n_z a multiple of the
+// block's sites).  STAGED is a template argument, not a runtime test: with
+// both psi sources in one body the compiler merged the LDS and global reads
+// into flat loads, each waiting for all memory traffic of the wave.
+template <class Store, bool SOLID, bool OFF32, bool STAGED = false>
 __device__ inline void ssplit_collide_block(const Store *__restrict__ A, Store *__restrict__ B,
                                             const uint32_t *__restrict__ nbw,
                                             const double *__restrict__ S,
@@ -427,19 +452,21 @@ __device__ inline void ssplit_collide_block(const Store *__restrict__ A, Store *
                                             float (*rows)[9][kShiftThreads / 2 + 2]) {
   using L = D3Q19;
   constexpr int SB = kShiftThreads / 2;
-  const bool staged = rows && g.nz % SB == 0;
+  constexpr bool staged = STAGED;
   const int w = threadIdx.x >> 6;
   const int f = w & 1;
   const int s = blk * (kShiftThreads / 2) + (w >> 1) * 64 + (threadIdx.x & 63);
   const int k = ((threadIdx.x >> 7) << 6) | (threadIdx.x & 63);
   const uint32_t sw = s >= g.sites ? (1u << 31) : (SOLID ? nbw[s] : 0u);
   const ShiftLane pl = shift_lane(p, f);
-  int x = 0, y = 0, z = 0;
+  NbrOffsets o{};
   float n[19];
   float rho = 0.0f, j[3] = {0.0f, 0.0f, 0.0f};
   if (!(sw >> 31)) {
+    int x, y, z;
     coords(s, g, x, y, z);
-    ssplit_pull<Store, SOLID, OFF32>(A + (int64_t)f * 19 * g.sp, g, s, x, y, z, sw, n);
+    o = nbr_offsets(g, x, y, z);
+    ssplit_pull<Store, SOLID, OFF32>(A + (int64_t)f * 19 * g.sp, g, s, o, sw, n);
   }
   if (staged) stage_psi_rows<SB, kShiftThreads, OFF32>(P, g, blk * SB, rows);
   if (!(sw >> 31)) {
@@ -457,10 +484,7 @@ __device__ inline void ssplit_collide_block(const Store *__restrict__ A, Store *
 #pragma unroll
     for (int d = 0; d < 3; ++d) j_p[d] = xch[f ^ 1][1 + d][k];
   }
-  auto nbr = [&](int i) {
-    return site_of(wrap(x + L::c[i][0], g.nx), wrap(y + L::c[i][1], g.ny),
-                   wrap(z + L::c[i][2], g.nz), g);
-  };
+  auto nbr = [&](int i) { return nbr_site(s, o, L::c[i][0], L::c[i][1], L::c[i][2]); };
   Store *__restrict__ Bf = B + (int64_t)f * 19 * g.sp;
   shifted_collide_site<SOLID>(
       f, n, rho, rho_p, j, j_p, sw, p, pl,
@@ -479,14 +503,15 @@ __global__ void __launch_bounds__(kShiftThreads)
   ssplit_psi_block<Store, SOLID, OFF32>(A, nbw, P, g, p, split_block(g));
 }
 
-template <class Store, bool SOLID, bool OFF32>
+template <class Store, bool SOLID, bool OFF32, bool STAGED>
 __global__ void __launch_bounds__(kShiftThreads)
     k_ssplit_collide(const Store *__restrict__ A, Store *__restrict__ B,
                      const uint32_t *__restrict__ nbw, const double *__restrict__ S,
                      const float *__restrict__ P, Geo g, ShiftParams p) {
   __shared__ float xch[2][4][kShiftThreads / 2];
-  __shared__ float rows[2][9][kShiftThreads / 2 + 2];
-  ssplit_collide_block<Store, SOLID, OFF32>(A, B, nbw, S, P, g, p, split_block(g), xch, rows);
+  __shared__ float rows[STAGED ? 2 : 1][9][kShiftThreads / 2 + 2];
+  ssplit_collide_block<Store, SOLID, OFF32, STAGED>(A, B, nbw, S, P, g, p, split_block(g), xch,
+                                                    rows);
 }
 
 template <class Store, bool OFF32>
@@ -503,12 +528,19 @@ inline void launch_ssplit_impl(const Store *a, Store *b, const uint32_t *nbw, co
       hipLaunchKernelGGL((k_ssplit_psi<Store, false, OFF32>), grid, block, 0, stream, a, nbw, P,
                          g, p);
   } else {
-    if (p.has_solid)
-      hipLaunchKernelGGL((k_ssplit_collide<Store, true, OFF32>), grid, block, 0, stream, a, b,
-                         nbw, S, P, g, p);
+    const bool staged = g.nz % (kShiftThreads / 2) == 0;
+    if (p.has_solid && staged)
+      hipLaunchKernelGGL((k_ssplit_collide<Store, true, OFF32, true>), grid, block, 0, stream, a,
+                         b, nbw, S, P, g, p);
+    else if (p.has_solid)
+      hipLaunchKernelGGL((k_ssplit_collide<Store, true, OFF32, false>), grid, block, 0, stream,
+                         a, b, nbw, S, P, g, p);
+    else if (staged)
+      hipLaunchKernelGGL((k_ssplit_collide<Store, false, OFF32, true>), grid, block, 0, stream,
+                         a, b, nbw, S, P, g, p);
     else
-      hipLaunchKernelGGL((k_ssplit_collide<Store, false, OFF32>), grid, block, 0, stream, a, b,
-                         nbw, S, P, g, p);
+      hipLaunchKernelGGL((k_ssplit_collide<Store, false, OFF32, false>), grid, block, 0, stream,
+                         a, b, nbw, S, P, g, p);
   }
 }
 
