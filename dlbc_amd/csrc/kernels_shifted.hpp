@@ -216,7 +216,8 @@ __device__ inline void shifted_collide_site(int f, const float *cur, float rho_c
 #pragma unroll
       for (int i = 1; i < Q; ++i) {
         const bool sn = SOLID && ((sw >> i) & 1u);
-        const float psi2 = sn ? psi_self : psi(f2, i);
+        const float psi_n = psi(f2, i);  // read before the test: no branch per read
+        const float psi2 = sn ? psi_self : psi_n;
         const float wp = ffmul(psi2, d3q19_wff(i));
 #pragma unroll
         for (int d = 0; d < 3; ++d)

@@ -211,7 +211,8 @@ __device__ inline void sc_force_offdiag(int f1, double (&F)[3], const StepParams
   if (cc != 0.0) {
 #pragma unroll
     for (int i = 1; i < L::q; ++i) {
-      const double psi2 = solid(i) ? psi(f2, -1) : psi(f2, i);
+      const double psi_n = psi(f2, i);  // read before the test, as in sc_force
+      const double psi2 = solid(i) ? psi(f2, -1) : psi_n;
       const double pf = L::w[i] * psi1 * psi2 * cc;
 #pragma unroll
       for (int d = 0; d < 3; ++d) F[d] -= pf * static_cast<double>(L::c[i][d]);
@@ -326,7 +327,10 @@ __device__ inline void collide_d3q19_streamed(int f, const double *n, double rho
   pair(tuyt - tuzt, 15, 10, false);
 }
 
-template <bool SOLID, bool OFFDIAG, bool OFF32>
+// STAGED (n_z a multiple of the block's sites): the psi rows come from LDS.
+// A template argument rather than a runtime test, so that the body holds one
+// psi source: with both, every psi read sat behind a branch and a full wait.
+template <bool SOLID, bool OFFDIAG, bool OFF32, bool STAGED>
 __global__ void __launch_bounds__(kSplitThreads)
     k_split_collide(const double *__restrict__ A, double *__restrict__ B,
                     const uint32_t *__restrict__ nbw, const double *__restrict__ S, Geo g,
@@ -337,8 +341,8 @@ __global__ void __launch_bounds__(kSplitThreads)
   // density and first moment of both components of the block's sites
   __shared__ double xch[2][4][SB];
   // psi rows of the block when it lies in one z-row
-  __shared__ double rows[2][9][SB + 2];
-  const bool staged = g.nz % SB == 0;
+  __shared__ double rows[STAGED ? 2 : 1][9][SB + 2];
+  constexpr bool staged = STAGED;
   int s, f;
   const bool inside = split_lane(g, s, f);
   const int k = ((threadIdx.x >> 7) << 6) | (threadIdx.x & 63);  // site within the block
@@ -556,6 +560,25 @@ inline void launch_ssplit_d3q19(const void *A, void *B, const uint32_t *nbw, con
     launch_ssplit_impl<Store, false>(a, b, nbw, S, P, g, p, stream, pass);
 }
 
+template <bool OFF32, bool STAGED>
+inline void launch_split_collide(const double *A, double *B, const uint32_t *nbw,
+                                 const double *S, const Geo &g, const StepParams &p,
+                                 hipStream_t stream, bool solid, bool offdiag) {
+  const dim3 grid(split_grid(g.sites)), block(kSplitThreads);
+  if (solid && offdiag)
+    hipLaunchKernelGGL((k_split_collide<true, true, OFF32, STAGED>), grid, block, 0, stream, A,
+                       B, nbw, S, g, p);
+  else if (solid)
+    hipLaunchKernelGGL((k_split_collide<true, false, OFF32, STAGED>), grid, block, 0, stream, A,
+                       B, nbw, S, g, p);
+  else if (offdiag)
+    hipLaunchKernelGGL((k_split_collide<false, true, OFF32, STAGED>), grid, block, 0, stream, A,
+                       B, nbw, S, g, p);
+  else
+    hipLaunchKernelGGL((k_split_collide<false, false, OFF32, STAGED>), grid, block, 0, stream,
+                       A, B, nbw, S, g, p);
+}
+
 template <bool OFF32>
 inline void launch_split_impl(const double *A, double *B, const uint32_t *nbw, double *S,
                               const Geo &g, const StepParams &p, hipStream_t stream, int pass) {
@@ -571,18 +594,10 @@ inline void launch_split_impl(const double *A, double *B, const uint32_t *nbw, d
     return;
   }
   const bool offdiag = p.gcc[0][0] == 0.0 && p.gcc[1][1] == 0.0;
-  if (p.has_solid && offdiag)
-    hipLaunchKernelGGL((k_split_collide<true, true, OFF32>), grid, block, 0, stream, A, B, nbw,
-                       S, g, p);
-  else if (p.has_solid)
-    hipLaunchKernelGGL((k_split_collide<true, false, OFF32>), grid, block, 0, stream, A, B, nbw,
-                       S, g, p);
-  else if (offdiag)
-    hipLaunchKernelGGL((k_split_collide<false, true, OFF32>), grid, block, 0, stream, A, B, nbw,
-                       S, g, p);
+  if (g.nz % (kSplitThreads / 2) == 0)
+    launch_split_collide<OFF32, true>(A, B, nbw, S, g, p, stream, p.has_solid != 0, offdiag);
   else
-    hipLaunchKernelGGL((k_split_collide<false, false, OFF32>), grid, block, 0, stream, A, B,
-                       nbw, S, g, p);
+    launch_split_collide<OFF32, false>(A, B, nbw, S, g, p, stream, p.has_solid != 0, offdiag);
 }
 
 inline void launch_split_d3q19(const double *A, double *B, const uint32_t *nbw, double *S,

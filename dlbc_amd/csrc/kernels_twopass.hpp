@@ -102,7 +102,10 @@ __device__ inline void sc_force(int f1, double (&F)[L::d], const StepParams &p, 
     if (cc == 0.0) continue;
 #pragma unroll
     for (int i = 1; i < Q; ++i) {  // Iota!(1, q-1): i = 1 .. Q-1
-      const double psi2 = solid(i) ? psi(f2, -1) : psi(f2, i);
+      // read before the test (every neighbour index is in bounds), so that
+      // the reads are not each behind a branch
+      const double psi_n = psi(f2, i);
+      const double psi2 = solid(i) ? psi(f2, -1) : psi_n;
       const double pf = L::w[i] * psi1 * psi2 * cc;
 #pragma unroll
       for (int d = 0; d < D; ++d) F[d] -= pf * static_cast<double>(L::c[i][d]);
