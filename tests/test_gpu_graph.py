@@ -44,8 +44,31 @@ def _walls(n, form="BDist2"):
                 fluidPerturb=[0.05, 0.05], maskArray=mask)
 
 
+def _d1q5_plates():
+    return dict(d=1, q=5, gn=[64], components=2, tau=[1.0, 0.8], eqDistForm="ThirdOrder",
+                gcc=[[0.0, 3.6], [3.6, 0.0]], enableShanChen=True, psiForm="Exponential",
+                fluidInit=["EqDistPerturbFrac"] * 2, fluidDensities=[[0.7], [0.5]],
+                fluidPerturb=[0.2, 0.2], randomReseed=[True, True], randomSeed=[5, 6],
+                maskInit="Plates", maskAxis="X", wallOffset=3, gwc=[0.1, -0.2],
+                globalAcc=[1e-4])
+
+
+def _three_components(n=(7, 6, 5)):
+    rng = np.random.default_rng(33)
+    mask = (rng.random(n) < 0.1).astype(np.int32)
+    return dict(d=3, q=19, gn=list(n), components=3, tau=[0.9, 1.1, 1.2],
+                eqDistForm="SecondOrder", gcc=[[0.5, 1.2, 0.8], [1.2, 0.0, 1.5],
+                                               [0.8, 1.5, 0.3]],
+                gwc=[0.1, -0.1, 0.05], enableShanChen=True, psiForm="Exponential",
+                globalAcc=[1e-5, 0.0, 0.0], fluidInit=["EqDistPerturb"] * 3,
+                fluidDensities=[[0.3], [0.5], [0.4]], fluidPerturb=[0.02] * 3,
+                randomReseed=[True] * 3, randomSeed=[21, 22, 23], maskArray=mask)
+
+
 # (id, parameters, storage, schedule)
 CASES = [
+    ("d1q5-plates", _d1q5_plates(), "f64", None),
+    ("d3q19-3comp-twopass", _three_components(), "f64", None),
     ("d2q9-bgk", _d2q9(), "f64", None),
     ("d3q19-split", _spinodal((16, 20, 24)), "f64", None),
     ("d3q19-split-zrows", _spinodal((6, 7, 256)), "f64", None),
@@ -78,9 +101,12 @@ def test_graph_replays_equal_eager_steps(D, name, p, storage, schedule):
         if L.nc == 1:
             L.set_params([0.8])
         else:
-            L.set_params([1.2, 0.95], [[0.0, 4.0], [4.0, 0.0]],
-                         [0.1, -0.1] if "walls" in name else None, None, None, "BDist2",
-                         "Exponential", True)
+            nc = L.nc
+            gcc = [[0.0 if i == j else 3.0 + 0.5 * (i + j) for j in range(nc)]
+                   for i in range(nc)]
+            L.set_params([1.2 - 0.1 * i for i in range(nc)], gcc,
+                         [0.1 * (-1) ** i for i in range(nc)] if p.get("gwc") else None, None,
+                         None, "BDist2" if L.d > 1 else "ThirdOrder", "Exponential", True)
     a.step(66, graph=True)
     b.step(66)
     _same(a, b, "after set_params")
