@@ -584,7 +584,11 @@ int do_steps(dlbc_gpu_ctx *c, int nsteps, Timer *tm) {
   return DLBC_OK;
 }
 
+// Replays may still be in flight on the stream: wait for them before the
+// executable graphs go.
 void drop_graphs(dlbc_gpu_ctx *c) {
+  if (!c->graph[0] && !c->graph[1]) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto &g : c->graph) {
     if (g) (void)hipGraphExecDestroy(g);
     g = nullptr;
@@ -1487,7 +1491,9 @@ int bfail(dlbc_batch *b, int code, const char *fmt, ...) {
   } while (0)
 
 void batch_drop_graph(dlbc_batch *b) {
-  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  if (!b->graph) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);  // replays may be in flight
+  (void)hipGraphExecDestroy(b->graph);
   b->graph = nullptr;
 }
 
