@@ -143,14 +143,15 @@ def test_batch_refuses_mixed_members(D):
             D.Batch(members)
 
 
-@pytest.mark.parametrize("n", [(20, 27, 45), (6, 7, 256)], ids=["unaligned", "z-rows"])
+@pytest.mark.parametrize("n", [(20, 27, 45), (6, 7, 256), (4, 5, 512)],
+                         ids=["unaligned", "z-rows", "z-rows-512"])
 @pytest.mark.parametrize("storage", ["f32_shift", "f16_shift"])
 @pytest.mark.parametrize("walls", [False, True], ids=["droplet", "walls"])
 def test_shifted_split_schedule_equals_tiled(D, storage, walls, n):
     """Shifted storage under the two-pass split schedule gives the same bits
     as the tiled kernel (psi through the P array instead of an LDS ring; the
     per-site arithmetic is shared), with walls, random solids, wall coupling
-    and a body force: on an unaligned lattice, and on one whose z-rows hold
+    and a body force: on an unaligned lattice, and on ones whose z-rows hold
     whole blocks, where the split kernel stages the psi rows in LDS."""
     p = droplet(n, 1.2, 3.8, 0.3)
     if walls:

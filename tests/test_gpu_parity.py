@@ -428,13 +428,14 @@ def test_split_schedule_matches_oracle(D, oracle_mod, form, walls, n):
     _same_state(g, o, "split %s walls=%s" % (form, walls))
 
 
+@pytest.mark.parametrize("n", [[10, 24, 40], [9, 10, 256]], ids=["gather", "staged"])
 @pytest.mark.parametrize("storage", ["f64", "f32_shift", "f16_shift"])
-def test_split_block_order_does_not_change_results(D, storage):
+def test_split_block_order_does_not_change_results(D, storage, n):
     """The XCD block order only permutes which workgroup takes which sites:
     site order, the automatic order, and chunks of 1, 3 and 16 blocks per
-    XCD turn give the same populations bit for bit, on a lattice whose grid
-    is not a whole number of turns (the tail keeps site order)."""
-    n = [10, 24, 40]
+    XCD turn give the same populations bit for bit, on lattices whose grid
+    is not a whole number of turns (the tail keeps site order), with psi
+    gathered per lane and staged in LDS."""
     p = _walls_case(n, "BDist2")
     pops = []
     for order in (0, -1, 1, 3, 16):
