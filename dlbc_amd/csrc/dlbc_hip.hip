@@ -1172,7 +1172,7 @@ int dlbc_gpu_step_graph(dlbc_gpu_ctx *c, int nsteps) {
     // the tiled kernel reads its parameters from the device table: refresh it
     // before every replay, not only before a capture
     rc = shifted_self_ready(c);
-    if (rc) return fail(c, rc, "shifted tiled kernel: cannot prepare its table");
+    if (rc) return rc;
   }
   auto key = graph_key(c, sched);
   if (key != c->graph_key) {
