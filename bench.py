@@ -419,6 +419,50 @@ def ensemble_workload(D, count, steps, storage="f32_shift"):
                                   "L ~ U{32..96}" % count, "results": res}), flush=True)
 
 
+def small_workload(D, reps=3):
+    """Small lattices, whose steps are bound by launch latency: eager steps
+    against graph replay (dlbc_gpu_step_graph, same bits), best of `reps`
+    timings each.  D2Q9 64^2 single-component BGK (rho 1 +- 1e-3, seed 42) and
+    256^2; D3Q19 two-component Shan-Chen spinodal decomposition (rho 0.7 +-
+    0.01, g 3.6, BDist2) at 32^3, 64^3 and 128^3."""
+    def d2q9(n):
+        return dict(d=2, q=9, gn=[n, n], components=1, tau=[1.0], fluidInit=["EqDistPerturb"],
+                    fluidDensities=[[1.0]], fluidPerturb=[1e-3], randomReseed=[True],
+                    randomSeed=[42])
+
+    def spinodal(n):
+        return dict(d=3, q=19, gn=[n, n, n], components=2, tau=[1.0, 1.0], eqDistForm="BDist2",
+                    gcc=[[0.0, 3.6], [3.6, 0.0]], enableShanChen=True, psiForm="Exponential",
+                    fluidInit=["EqDistPerturb"] * 2, fluidDensities=[[0.7], [0.7]],
+                    fluidPerturb=[0.01, 0.01], randomReseed=[True, True], randomSeed=[42, 43])
+    cases = [("D2Q9 64^2 BGK", d2q9(64), 1000), ("D2Q9 256^2 BGK", d2q9(256), 1000),
+             ("D3Q19 32^3 two-fluid SC", spinodal(32), 1000),
+             ("D3Q19 64^3 two-fluid SC", spinodal(64), 500),
+             ("D3Q19 128^3 two-fluid SC", spinodal(128), 200)]
+    out = []
+    for name, p, steps in cases:
+        L = D.Lattice.from_params(p)
+        sites = int(np.prod(p["gn"]))
+        res = {"lattice": name, "steps": steps}
+        for graph in (False, True):
+            L.step(2 * 32, graph=graph)  # warm-up (and the captures)
+            L.synchronize()
+            best = float("inf")
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                L.step(steps, graph=graph)
+                L.synchronize()
+                best = min(best, time.perf_counter() - t0)
+            key = "graph" if graph else "eager"
+            res[key + "_mlups"] = round(sites * steps / best / 1e6, 1)
+            res[key + "_us_per_step"] = round(best / steps * 1e6, 2)
+        out.append(res)
+        log("%s: eager %.1f, graph %.1f MLUPS" % (name, res["eager_mlups"], res["graph_mlups"]))
+        del L
+    print(json.dumps({"workload": "small lattices, eager steps against graph replay",
+                      "results": out}), flush=True)
+
+
 def copy_bandwidth(gib=2.0, reps=5):
     """A plain streaming kernel of known traffic (torch.neg over FP64 of
     `gib` GiB: every byte read once and written once), for scaling the
@@ -459,11 +503,12 @@ def main():
                     help="steps after which the shifted storages are compared with FP64 "
                          "(0: no comparison)")
     ap.add_argument("--workload", default="channel",
-                    choices=["channel", "droplet", "ensemble", "slab"],
+                    choices=["channel", "droplet", "ensemble", "slab", "small"],
                     help="channel: the MLUPS line; droplet: shifted-storage errors against "
                          "FP64 on a centred droplet (radius 96 at 512^3, scaled with --size); "
                          "ensemble: 256 uneven lattices, batched vs looped; slab: one GPU's "
-                         "x-slab of the channel")
+                         "x-slab of the channel; small: small lattices, eager steps against "
+                         "graph replay")
     ap.add_argument("--lattices", type=int, default=256, help="ensemble size")
     ap.add_argument("--slab-planes", type=int, default=32,
                     help="slab workload: x-planes of one GPU's share")
@@ -489,6 +534,9 @@ def main():
         return
     if args.workload == "slab":
         slab_workload(D, args.size, args.slab_planes, args.steps, args.warmup)
+        return
+    if args.workload == "small":
+        small_workload(D)
         return
     global BLOCK_ORDER
     BLOCK_ORDER = args.block_order

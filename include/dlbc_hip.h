@@ -113,7 +113,7 @@ int dlbc_gpu_step(dlbc_gpu_ctx *ctx, int nsteps);
 /* As dlbc_gpu_step, with the steps replayed from a captured HIP graph of 32
  * steps (one graph per buffer parity; the remainder of nsteps runs eagerly).
  * Same results bit for bit; for small lattices, whose steps are bound by
- * launch latency (D2Q9 64x64: 3.8 instead of 4.5 us per step).  The graphs are
+ * launch latency (D2Q9 64x64: 3.9 instead of 4.7 us per step).  The graphs are
  * captured on first use and again when the schedule, launch shape, block
  * order, parameters or solid state changed since.
  * Replaces the same runTimeloop body as dlbc_gpu_step (src/main.d:174-200). */
