@@ -146,3 +146,35 @@ def test_shifted_storage_shape_checks(d, q, nc, n):
     import dlbc_amd as D
     with pytest.raises(D.DlbcError, match="unsupported"):
         D.Lattice(d, q, nc, n, storage="f32_shift")
+
+
+def test_invalid_arguments_are_rejected_on_the_host():
+    """Entry points check their arguments before any device work: a null
+    context (or batch) gives DLBC_ERR_ARG, and the library never exits."""
+    import ctypes
+    import dlbc_amd
+    lib = ctypes.CDLL(dlbc_amd.lib_path())
+    ERR_ARG = -1
+    for name, args in [("dlbc_gpu_step", (None, 1)), ("dlbc_gpu_step_graph", (None, 1)),
+                       ("dlbc_gpu_advect", (None,)), ("dlbc_gpu_synchronize", (None,)),
+                       ("dlbc_gpu_set_launch", (None, 0)),
+                       ("dlbc_gpu_set_block_order", (None, 0)),
+                       ("dlbc_gpu_set_schedule", (None, 0)),
+                       ("dlbc_batch_step", (None, 1, 1))]:
+        fn = getattr(lib, name)
+        fn.restype = ctypes.c_int
+        assert fn(*args) == ERR_ARG, name
+    out = ctypes.c_void_p()
+    create = lib.dlbc_gpu_create
+    create.restype = ctypes.c_int
+    n = (ctypes.c_int64 * 3)(8, 8, 8)
+    ERR_UNSUPPORTED = -4
+    for d, q, comps, storage, rc in [(3, 15, 2, 0, ERR_UNSUPPORTED), (3, 19, 0, 0, ERR_UNSUPPORTED),
+                                     (3, 19, 5, 0, ERR_UNSUPPORTED), (3, 19, 2, 7, ERR_ARG),
+                                     (2, 9, 2, 1, ERR_UNSUPPORTED)]:
+        assert create(ctypes.byref(out), d, q, comps, n, storage, 0) == rc, (d, q, comps, storage)
+        assert out.value is None
+    small = (ctypes.c_int64 * 3)(1, 8, 8)  # below the reference's two periodic images
+    assert create(ctypes.byref(out), 3, 19, 2, small, 0, 0) == ERR_ARG
+    d1q5 = (ctypes.c_int64 * 1)(3)  # D1Q5 needs n >= 4 (halo 4)
+    assert create(ctypes.byref(out), 1, 5, 2, d1q5, 0, 0) == ERR_ARG
