@@ -300,7 +300,13 @@ def droplet_workload(D, n, steps, storages=("f16_shift", "f32_shift")):
         L.step(steps)
         L.synchronize()
         dt = time.perf_counter() - t
+        # whole-step algorithmic bytes: populations read and written once per
+        # site (no solids, so no mask byte), over the nominal HBM peak
+        elem = {"f64": 8, "f32_shift": 4, "f16_shift": 2}[storage]
+        step_GBps = n ** 3 * 2 * C * Q * elem * steps / dt / 1e9
         out = {"mlups": round(n ** 3 * steps / dt / 1e6, 1), "seconds": round(dt, 2),
+               "step_alg_GBps": round(step_GBps, 1),
+               "step_frac": round(step_GBps * 1e9 / HBM_PEAK, 4),
                "radius_t0": lap0["radius"], "laplace": L.laplace(),
                "mass_rel_drift": max(abs(L.total_mass(f) - m0[f]) / m0[f] for f in range(2))}
         log("%s: %s" % (storage, out))
@@ -386,6 +392,7 @@ def ensemble_workload(D, count, steps, storage="f32_shift"):
     sum(L^3) per step."""
     import torch
     params = ensemble_params(count)
+    elem_bytes = {"f64": 8, "f32_shift": 4, "f16_shift": 2}[storage]
     t0 = time.perf_counter()
     lats = [D.Lattice.from_params(p, storage=storage) for p in params]
     sites = sum(int(np.prod(p["gn"])) for p in params)
@@ -412,6 +419,10 @@ def ensemble_workload(D, count, steps, storage="f32_shift"):
                       "kernel": ["tiled, solid-capable", "tiled, no-solid", "split"][cfg[3]]},
            "autotune_launch_ms": round(ms, 4),
            "batched_graph_mlups": round(sites * steps / dt_graph / 1e6, 1),
+           # whole-step algorithmic bytes (populations read and written once per
+           # site; these lattices have no solids) over the nominal HBM peak
+           "batched_graph_step_frac": round(sites * steps * 2 * C * Q * elem_bytes / dt_graph
+                                            / HBM_PEAK, 4),
            "batched_eager_mlups": round(sites * steps / dt_eager / 1e6, 1),
            "looped_mlups": round(sites * steps / dt_loop / 1e6, 1),
            "speedup_vs_looped": round(dt_loop / dt_graph, 3), "finite": bool(finite)}
