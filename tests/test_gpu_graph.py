@@ -164,3 +164,25 @@ def test_graph_after_mask_change(D):
     a.step(36, graph=True)
     b.step(36)
     _same(a, b, "after init_mask")
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32_shift"])
+def test_graph_after_restore(D, storage):
+    """A restart in the middle of a run (the populations of another lattice
+    uploaded, as from a checkpoint) with the graphs already captured: the
+    replays read the new state, as eager steps do."""
+    p = _walls((16, 16, 32))
+    a = D.Lattice.from_params(p, storage=storage)
+    b = D.Lattice.from_params(p, storage=storage)
+    c = D.Lattice.from_params(p, storage=storage)
+    a.step(40, graph=True)  # a holds captured graphs for both buffer parities
+    a.step(33, graph=True)
+    b.step(73)
+    c.step(19)  # the state to restore
+    state = [c.population(f) for f in range(2)]
+    for L in (a, b):
+        for f in range(2):
+            L.upload_population(f, state[f])
+    a.step(70, graph=True)
+    b.step(70)
+    _same(a, b, "after restore")
