@@ -435,21 +435,24 @@ def small_workload(D, reps=3):
     against graph replay (dlbc_gpu_step_graph, same bits), best of `reps`
     timings each.  D2Q9 64^2 single-component BGK (rho 1 +- 1e-3, seed 42) and
     256^2; D3Q19 two-component Shan-Chen spinodal decomposition (rho 0.7 +-
-    0.01, g 3.6, BDist2) at 32^3, 64^3 and 128^3."""
+    0.01, g 3.6, BDist2) at 32^3, 64^3 and 128^3, and at 128^3 with the
+    SecondOrder equilibrium for 1000 steps."""
     def d2q9(n):
         return dict(d=2, q=9, gn=[n, n], components=1, tau=[1.0], fluidInit=["EqDistPerturb"],
                     fluidDensities=[[1.0]], fluidPerturb=[1e-3], randomReseed=[True],
                     randomSeed=[42])
 
-    def spinodal(n):
-        return dict(d=3, q=19, gn=[n, n, n], components=2, tau=[1.0, 1.0], eqDistForm="BDist2",
+    def spinodal(n, form="BDist2"):
+        return dict(d=3, q=19, gn=[n, n, n], components=2, tau=[1.0, 1.0], eqDistForm=form,
                     gcc=[[0.0, 3.6], [3.6, 0.0]], enableShanChen=True, psiForm="Exponential",
                     fluidInit=["EqDistPerturb"] * 2, fluidDensities=[[0.7], [0.7]],
                     fluidPerturb=[0.01, 0.01], randomReseed=[True, True], randomSeed=[42, 43])
     cases = [("D2Q9 64^2 BGK", d2q9(64), 1000), ("D2Q9 256^2 BGK", d2q9(256), 1000),
              ("D3Q19 32^3 two-fluid SC", spinodal(32), 1000),
              ("D3Q19 64^3 two-fluid SC", spinodal(64), 500),
-             ("D3Q19 128^3 two-fluid SC", spinodal(128), 200)]
+             ("D3Q19 128^3 two-fluid SC", spinodal(128), 200),
+             ("D3Q19 128^3 two-fluid SC, SecondOrder (the spinodal workload as specified)",
+              spinodal(128, "SecondOrder"), 1000)]
     out = []
     for name, p, steps in cases:
         L = D.Lattice.from_params(p)
