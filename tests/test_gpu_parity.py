@@ -578,3 +578,21 @@ def test_restart_from_hdf5_checkpoint(D, tmp_path):
     back, _ = h5io.read_field(h5io.output_name(str(tmp_path), "colour-red-blue", sim, 5))
     assert_close(back, L.colour(), 0.0, "output colour")
     assert len(files) == 5
+
+
+def test_bench_channel_workload_bitexact_at_64(D, oracle_mod):
+    """The bench.py channel workload (walls at z = 0 and n - 1, seeded random
+    solid spheres to a solid fraction of 0.2, red/blue lamellae along x,
+    BDist2, g 3.6) at 64^3 on the default schedule, 20 steps: bit-identical
+    to the oracle.  The body force is left out: the oracle has none (the
+    reference adds it from a plugin hook); it is checked on its own above."""
+    import bench
+    n = 64
+    p = bench.channel_params(n)
+    p.pop("bodyForce")
+    p["maskArray"] = bench.channel_mask(n)
+    g, o = _pair(D, oracle_mod, p)
+    assert g.schedule == "split"
+    g.step(20)
+    o.run(20)
+    _same_state(g, o, "channel 64^3")
