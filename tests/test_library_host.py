@@ -178,3 +178,8 @@ def test_invalid_arguments_are_rejected_on_the_host():
     assert create(ctypes.byref(out), 3, 19, 2, small, 0, 0) == ERR_ARG
     d1q5 = (ctypes.c_int64 * 1)(3)  # D1Q5 needs n >= 4 (halo 4)
     assert create(ctypes.byref(out), 1, 5, 2, d1q5, 0, 0) == ERR_ARG
+    huge = (ctypes.c_int64 * 3)(2048, 1024, 1024)  # 2^31 sites: beyond the 2^30-site limit
+    assert create(ctypes.byref(out), 3, 19, 2, huge, 0, 0) == ERR_UNSUPPORTED
+    long_axis = (ctypes.c_int64 * 3)((1 << 20) + 1, 2, 2)  # an axis beyond 2^20
+    assert create(ctypes.byref(out), 3, 19, 2, long_axis, 0, 0) == ERR_ARG
+    assert out.value is None
