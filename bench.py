@@ -225,6 +225,7 @@ def cpu_baseline(p, mask, pops, seconds=10.0, max_steps=60):
     n = p["gn"]
     o = OR.Oracle(3, Q, 2, n)
     o.set_params(p["tau"], p["gcc"], None, None, 2, 1, True)
+    o.set_body_force(p.get("bodyForce"))  # the reference's pre-collision hook plugin
     o.set_mask(mask)
     for f in range(2):
         o.set_population(f, pops[f])
@@ -248,8 +249,8 @@ def cpu_baseline(p, mask, pops, seconds=10.0, max_steps=60):
             "nproc": os.cpu_count(),
             "single_core_value": round(float(np.prod(n)) / dt1 / 1e6, 3),
             "sample": "%d steps of the same %s workload (AoS + halo 2, reference pass "
-                      "order, OpenMP); the body force, which the reference adds from a "
-                      "plugin hook, is left out of this leg" % (steps, "x".join(map(str, n)))}
+                      "order, OpenMP), the body force added as the reference's "
+                      "pre-collision hook" % (steps, "x".join(map(str, n)))}
 
 
 def droplet_params(n, radius):

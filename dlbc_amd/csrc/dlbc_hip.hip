@@ -163,6 +163,40 @@ __global__ void k_soa_to_aos(Ld ld, double *__restrict__ stg, int q, int f, int 
   for (int i = 0; i < q; ++i) stg[(int64_t)t * q + i] = ld(f, i, s0 + t);
 }
 
+// D default-initialises a double to the quiet NaN 0x7FFC000000000000, and a
+// component initialised with FluidInit.None keeps it (src/dlbc/lb/init.d:
+// 263-270; the lb/init-2d reference data hold it).  Every population buffer
+// starts out so: FP32 0x7FE00000 and FP16 0x7F00 widen to the same double.
+template <class St>
+__device__ inline St d_double_init();
+template <>
+__device__ inline double d_double_init<double>() { return __longlong_as_double(0x7FFC000000000000ll); }
+template <>
+__device__ inline float d_double_init<float>() { return __int_as_float(0x7FE00000); }
+template <>
+__device__ inline __half d_double_init<__half>() { return __ushort_as_half((unsigned short)0x7F00); }
+
+template <class St>
+__global__ void k_fill_default(St *__restrict__ A, St *__restrict__ B, int64_t n) {
+  const St v = d_double_init<St>();
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    A[i] = v;
+    B[i] = v;
+  }
+}
+
+// The step kernels never write a solid site: its populations stay as they are,
+// as in the reference (advection.d:84-86), provided both buffers hold the same
+// values there.  After a mask change mid-run, copy the current buffer A into B
+// at every solid site.
+template <class St>
+__global__ void k_copy_solid(const St *__restrict__ A, St *__restrict__ B,
+                             const uint8_t *__restrict__ mask, Geo g, int nq) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= g.sites || !mask[s]) return;
+  for (int k = 0; k < nq; ++k) B[(int64_t)k * g.sp + s] = A[(int64_t)k * g.sp + s];
+}
+
 // Solid-neighbour word of every site: bit i = solid(x + c_i), bit 31 =
 // solid(x) (periodic neighbours).
 template <class L>
@@ -828,6 +862,20 @@ int dlbc_gpu_create(dlbc_gpu_ctx **out, int d, int q, int components, const int6
     return bail(DLBC_ERR_HIP);
   if (hipMemsetAsync(c->S, 0, sizeof(double) * components * c->geo.sp, c->stream) != hipSuccess)
     return bail(DLBC_ERR_HIP);
+  {
+    const int64_t nel = (int64_t)components * q * c->geo.sp;
+    const dim3 grid((unsigned)std::min<int64_t>(blocks_for(nel, 256), 8192)), block(256);
+    if (storage == DLBC_STORAGE_F64)
+      hipLaunchKernelGGL(k_fill_default<double>, grid, block, 0, c->stream,
+                         static_cast<double *>(c->popA), static_cast<double *>(c->popB), nel);
+    else if (storage == DLBC_STORAGE_F32_SHIFT)
+      hipLaunchKernelGGL(k_fill_default<float>, grid, block, 0, c->stream,
+                         static_cast<float *>(c->popA), static_cast<float *>(c->popB), nel);
+    else
+      hipLaunchKernelGGL(k_fill_default<__half>, grid, block, 0, c->stream,
+                         static_cast<__half *>(c->popA), static_cast<__half *>(c->popB), nel);
+    if (hipGetLastError() != hipSuccess) return bail(DLBC_ERR_HIP);
+  }
   for (auto &e : c->ev)
     if (hipEventCreate(&e) != hipSuccess) return bail(DLBC_ERR_HIP);
   c->events_ready = true;
@@ -862,8 +910,18 @@ int dlbc_gpu_set_params(dlbc_gpu_ctx *c, const double *tau, const double *gcc, c
   p.psi_form = psi_form;
   p.sc = enable_shan_chen ? 1 : 0;
   p.has_solid = c->has_solid;
+  // Shifted storage keeps f_i - w_i rho0_f: once a component's populations are
+  // on the device its rho0 is part of their encoding and cannot change (NULL
+  // keeps the current values, 1.0 until first set).
+  if (rho0)
+    for (int f = 0; f < c->C; ++f)
+      if (c->storage != DLBC_STORAGE_F64 && c->pops_uploaded[f] && rho0[f] != c->rho0[f])
+        return fail(c, DLBC_ERR_STATE,
+                    "rho0 of component %d cannot change once its populations are on the "
+                    "device (%g stored, %g given)", f, c->rho0[f], rho0[f]);
   c->p = p;
-  for (int f = 0; f < c->C; ++f) c->rho0[f] = rho0 ? rho0[f] : 1.0;
+  if (rho0)
+    for (int f = 0; f < c->C; ++f) c->rho0[f] = rho0[f];
   if (c->storage != DLBC_STORAGE_F64) {
     ShiftParams sp{};
     for (int f = 0; f < 2; ++f) {
@@ -922,6 +980,23 @@ static int mask_ready(dlbc_gpu_ctx *c, int solid) {
     else
       hipLaunchKernelGGL(k_solid_words<D1Q3>, dim3(nb), dim3(256), 0, c->stream, c->mask, c->nbw,
                          c->geo);
+    HIPCHK(c, hipGetLastError());
+  }
+  if (solid) {
+    const int nb = blocks_for(c->geo.sites, 256);
+    const int nq = c->C * c->q;
+    if (c->storage == DLBC_STORAGE_F64)
+      hipLaunchKernelGGL(k_copy_solid<double>, dim3(nb), dim3(256), 0, c->stream,
+                         static_cast<const double *>(c->popA), static_cast<double *>(c->popB),
+                         c->mask, c->geo, nq);
+    else if (c->storage == DLBC_STORAGE_F32_SHIFT)
+      hipLaunchKernelGGL(k_copy_solid<float>, dim3(nb), dim3(256), 0, c->stream,
+                         static_cast<const float *>(c->popA), static_cast<float *>(c->popB),
+                         c->mask, c->geo, nq);
+    else
+      hipLaunchKernelGGL(k_copy_solid<__half>, dim3(nb), dim3(256), 0, c->stream,
+                         static_cast<const __half *>(c->popA), static_cast<__half *>(c->popB),
+                         c->mask, c->geo, nq);
     HIPCHK(c, hipGetLastError());
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1150,6 +1225,18 @@ int dlbc_gpu_download_populations(dlbc_gpu_ctx *c, int f, double *host, int halo
         o += c->n[2];
       }
   }
+  return DLBC_OK;
+}
+
+int dlbc_gpu_download_mask(dlbc_gpu_ctx *c, int32_t *host, int halo) {
+  if (!c || !host || halo < 0) return DLBC_ERR_ARG;
+  std::vector<uint8_t> m((size_t)c->geo.sites);
+  HIPCHK(c, hipMemcpyAsync(m.data(), c->mask, m.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  size_t o = 0;
+  for (int x = 0; x < c->n[0]; ++x)
+    for (int y = 0; y < c->n[1]; ++y)
+      for (int z = 0; z < c->n[2]; ++z) host[host_index(c, halo, x, y, z)] = m[o++];
   return DLBC_OK;
 }
 
@@ -1468,6 +1555,7 @@ struct dlbc_batch {
   std::vector<int> h_first;
   bool items_dirty = true;
   hipGraphExec_t graph = nullptr;
+  int graph_sig = -1;  // batch_launch_sig() when the graph was captured
   std::string err;
 };
 
@@ -1526,6 +1614,26 @@ int batch_build_items(dlbc_batch *b) {
   return DLBC_OK;
 }
 
+bool batch_has_solid(const dlbc_batch *b) {
+  for (auto *c : b->ctx)
+    if (c->has_solid) return true;
+  return false;
+}
+
+bool batch_has_sc(const dlbc_batch *b, int l0, int l1) {
+  for (int l = l0; l < l1; ++l)
+    if (b->ctx[l]->sparams.sc) return true;
+  return false;
+}
+
+// The host state that picks kernels at launch beyond the configuration: any
+// member with solids (the tiled kernel's SOLID variant) and any with
+// Shan-Chen (the split schedule's psi pass).  A captured graph is only valid
+// for the value it was captured with.
+int batch_launch_sig(const dlbc_batch *b) {
+  return (batch_has_solid(b) ? 1 : 0) | (batch_has_sc(b, 0, (int)b->ctx.size()) ? 2 : 0);
+}
+
 // Make every member ready (solid densities, pending uploads done) and the
 // device table current (buffers swap between calls, parameters may change).
 int batch_prepare(dlbc_batch *b) {
@@ -1551,24 +1659,20 @@ int batch_prepare(dlbc_batch *b) {
     b->h_lats = t;
     b->lats_valid = true;
   }
+  if (b->graph && b->graph_sig != batch_launch_sig(b)) batch_drop_graph(b);
   if (b->items_dirty) return batch_build_items(b);
   return DLBC_OK;
-}
-
-bool batch_has_solid(const dlbc_batch *b) {
-  for (auto *c : b->ctx)
-    if (c->has_solid) return true;
-  return false;
 }
 
 // Split schedule over the members [l0, l1).
 int batch_launch_split(dlbc_batch *b, int l0, int l1, int parity) {
   const int nlat = (int)b->ctx.size();
   const int blk0 = b->h_first[l0], nblk = b->h_first[l1] - blk0;
+  const bool psi = batch_has_sc(b, l0, l1);
   if (b->storage == DLBC_STORAGE_F32_SHIFT)
-    launch_batch_split<float>(b->d_lats, b->d_first, nlat, blk0, nblk, parity, b->stream);
+    launch_batch_split<float>(b->d_lats, b->d_first, nlat, blk0, nblk, parity, psi, b->stream);
   else
-    launch_batch_split<__half>(b->d_lats, b->d_first, nlat, blk0, nblk, parity, b->stream);
+    launch_batch_split<__half>(b->d_lats, b->d_first, nlat, blk0, nblk, parity, psi, b->stream);
   BHIPCHK(b, hipGetLastError());
   return DLBC_OK;
 }
@@ -1685,6 +1789,7 @@ int dlbc_batch_step(dlbc_batch *b, int nsteps, int use_graph) {
       const hipError_t ei = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       BHIPCHK(b, ei);
+      b->graph_sig = batch_launch_sig(b);
     }
     for (; t + 2 <= nsteps; t += 2) BHIPCHK(b, hipGraphLaunch(b->graph, b->stream));
   }

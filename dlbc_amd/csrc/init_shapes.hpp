@@ -257,7 +257,11 @@ inline bool resolve_mask(MaskSpec &m, int d, const int64_t *n, int halo, int kin
                          int wall_offset, const int *dim_has_wall, int wall_thickness) {
   if (d < 1 || d > 3 || halo < 0 || !n) return false;
   if (kind != 0 && (kind < 2 || kind > 5)) return false;
-  if ((kind == 2 || kind == 3) && (axis < 0 || axis >= d)) return false;
+  // Plates needs an axis of the lattice (writeLogF, lb/mask.d:160-163); Tube
+  // takes any Axis value X, Y, Z and with one beyond d is a Box (the reference
+  // only warns, mask.d:141-143; lb/init-2d mask-init-tube-z-2d)
+  if (kind == 2 && (axis < 0 || axis >= d)) return false;
+  if (kind == 3 && (axis < 0 || axis > 2)) return false;
   if (kind == 5 && !dim_has_wall) return false;
   m = MaskSpec{};
   m.kind = kind;

@@ -361,6 +361,7 @@ __global__ void __launch_bounds__(kShiftThreads)
   const Store *A = static_cast<const Store *>(B.pop[parity]);
   const Geo g = B.g;
   const ShiftParams p = B.p;
+  if (!p.sc) return;  // this member's collide pass reads no psi
   if (p.has_solid)
     ssplit_psi_block<Store, true, false>(A, B.nbw, B.P, g, p, b - first[l]);
   else
@@ -388,14 +389,16 @@ __global__ void __launch_bounds__(kShiftThreads)
                                               nullptr);
 }
 
-// One step of the blocks [blk0, blk0 + nblk) of a batch on the split schedule.
+// One step of the blocks [blk0, blk0 + nblk) of a batch on the split schedule;
+// psi: whether any of these members has Shan-Chen on (else pass 1 is skipped).
 template <class Store>
 inline void launch_batch_split(const BatchLattice *lats, const int *first, int nlat, int blk0,
-                               int nblk, int parity, hipStream_t stream) {
+                               int nblk, int parity, bool psi, hipStream_t stream) {
   if (nblk <= 0) return;
   const int grid = nblk;
-  hipLaunchKernelGGL((k_batch_split_psi<Store>), dim3(grid), dim3(kShiftThreads), 0, stream,
-                     lats, first, nlat, blk0, blk0 + nblk, parity);
+  if (psi)
+    hipLaunchKernelGGL((k_batch_split_psi<Store>), dim3(grid), dim3(kShiftThreads), 0, stream,
+                       lats, first, nlat, blk0, blk0 + nblk, parity);
   hipLaunchKernelGGL((k_batch_split_collide<Store>), dim3(grid), dim3(kShiftThreads), 0, stream,
                      lats, first, nlat, blk0, blk0 + nblk, parity);
 }

@@ -54,6 +54,7 @@ def lib():
         "dlbc_gpu_set_params": (i, [vp, opt_dp, opt_dp, opt_dp, opt_dp, opt_dp, i, i, i, opt_dp]),
         "dlbc_gpu_upload_populations": (i, [vp, i, _dp, i]),
         "dlbc_gpu_upload_mask": (i, [vp, _ip, i]),
+        "dlbc_gpu_download_mask": (i, [vp, _ip, i]),
         "dlbc_gpu_step": (i, [vp, i]),
         "dlbc_gpu_step_graph": (i, [vp, i]),
         "dlbc_gpu_advect": (i, [vp]),
@@ -160,10 +161,15 @@ def _fluid_args(kind, densities, eq_dist_form, perturb, rng, radius, offset, sep
             float(radius), off, sep, int(axis), float(thickness), wid)
 
 
+# D's default-initialised double, the quiet NaN 0x7FFC000000000000: a field
+# starts out so, and FluidInit.None leaves it (lb/init-2d reference data).
+D_DOUBLE_INIT = np.array([0x7FFC000000000000], np.uint64).view(np.float64)[0]
+
+
 def init_fluid(d, q, n, kind, densities, eq_dist_form=0, perturb=0.0, rng=None, radius=0.0,
                offset=None, separation=None, axis=0, thickness=0.0, widths=None, halo=2):
     """initFluid (src/dlbc/lb/init.d:263-339) into a new halo-padded array."""
-    out = np.full(_padded_shape(n, halo) + (q,), np.nan)
+    out = np.full(_padded_shape(n, halo) + (q,), D_DOUBLE_INIT)
     a = _fluid_args(kind, densities, eq_dist_form, perturb, rng, radius, offset, separation, axis,
                     thickness, widths)
     rc = lib().dlbc_init_fluid(out.reshape(-1), d, q, np.asarray(n, np.int64), halo, *a)
@@ -332,6 +338,12 @@ class Lattice:
         out = np.zeros(_padded_shape(self.n, halo) + (self.q,))
         self._check(lib().dlbc_gpu_download_populations(self.h, f, out.reshape(-1), halo),
                     "download_populations")
+        return out
+
+    def mask(self, halo=0):
+        """The mask field (int32, 0 fluid, 1 solid) as the reference writes it."""
+        out = np.zeros(_padded_shape(self.n, halo), np.int32)
+        self._check(lib().dlbc_gpu_download_mask(self.h, out.reshape(-1), halo), "download_mask")
         return out
 
     def density(self, f):

@@ -87,8 +87,12 @@ int dlbc_gpu_create(dlbc_gpu_ctx **out, int d, int q, int components, const int6
  * body_force[C*d] per-component force density added to F_f before the
  *              velocity shift (no reference counterpart: the reference adds
  *              such a force from a preCollisionHooks plugin); NULL for none
- * rho0[C]      reference densities of the shifted storages (ignored for F64;
- *              NULL: 1.0) */
+ * rho0[C]      reference densities of the shifted storages (ignored for F64).
+ *              NULL keeps the current values (1.0 until first given).  Once a
+ *              component's populations are on the device its rho0 is part of
+ *              their encoding: a different value is refused with
+ *              DLBC_ERR_STATE (set rho0 before upload or device init).
+ * May be called again between steps to change tau, g, forces or forms. */
 int dlbc_gpu_set_params(dlbc_gpu_ctx *ctx, const double *tau, const double *gcc,
                         const double *gwc, const double *global_acc,
                         const double *body_force, int eq_dist_form, int psi_form,
@@ -96,14 +100,21 @@ int dlbc_gpu_set_params(dlbc_gpu_ctx *ctx, const double *tau, const double *gcc,
 
 /* Upload component f from a host array [n0+2h]..[q] (halo h >= 0; only the
  * physical sites are read).  Equivalent of initFluids / readCheckpoint
- * followed by exchangeHalo (src/main.d:136-147). */
+ * followed by exchangeHalo (src/main.d:136-147).  Until a component is
+ * uploaded or initialised its populations hold D's default double, the quiet
+ * NaN 0x7FFC000000000000, as the reference's fields do (FluidInit.None). */
 int dlbc_gpu_upload_populations(dlbc_gpu_ctx *ctx, int f, const double *host, int halo);
 
 /* Upload the mask [n0+2h].. int32, 0 = Mask.None (fluid), 1 = Mask.Solid
  * (src/dlbc/lb/mask.d:84-93).  Solid sites keep their populations forever
- * (advection copies them, collision skips them); upload the mask before the
- * populations. */
+ * (advection copies them, collision skips them), also when the mask changes
+ * between steps. */
 int dlbc_gpu_upload_mask(dlbc_gpu_ctx *ctx, const int32_t *host, int halo);
+
+/* Download the mask into a host array [n0+2h].. int32 (physical sites
+ * written, halo untouched): the mask field the reference writes as output and
+ * checkpoint (src/dlbc/lb/io.d:56-101 maskFreq, src/dlbc/io/checkpoint.d). */
+int dlbc_gpu_download_mask(dlbc_gpu_ctx *ctx, int32_t *host, int halo);
 
 /* Advance nsteps time steps: nsteps x the runTimeloop body
  * (src/main.d:174-200).  The state afterwards is post-collision, as the

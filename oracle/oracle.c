@@ -45,6 +45,8 @@ struct or_lattice {
   double gwc[OR_MAXC];
   double global_acc[3];
   int eq_dist_form, psi_form, enable_sc;
+  double body[OR_MAXC][3]; /* force density of a pre-collision hook plugin */
+  int has_body;
   double timers[7];
   int density_fresh; /* Field.isFresh of L.density (lb/density.d:344-362) */
 };
@@ -117,6 +119,16 @@ int or_conn(int d, int q, int *vel, int *bounce, double *w, double *css) {
 /* Lattice storage: src/dlbc/lattice.d:96-181, lb/init.d:60-73              */
 /* ------------------------------------------------------------------------ */
 
+/* The bit pattern of D's default-initialised double, 0x7FFC000000000000 (a
+ * quiet NaN), as the reference writes it for components initialised with
+ * FluidInit.None (lb/init-2d fixtures, e.g. population-red-mask-init-none-2d). */
+double or_d_double_init(void) {
+  const uint64_t bits = 0x7FFC000000000000ull;
+  double v;
+  memcpy(&v, &bits, sizeof v);
+  return v;
+}
+
 or_lattice *or_create(int d, int q, int ncomp, const int *n) {
   or_lattice *L;
   int i, vel[OR_MAXQ * 3];
@@ -159,16 +171,17 @@ or_lattice *or_create(int d, int q, int ncomp, const int *n) {
     L->density[i] = (double *)calloc(L->sites_h, sizeof(double));
     L->psi[i] = (double *)calloc(L->sites_h, sizeof(double));
     L->force[i] = (double *)calloc(L->sites_h * d, sizeof(double));
-    /* D initialises doubles to NaN (field.d: multidimArray default init) */
+    /* D initialises doubles to its default NaN (field.d: multidimArray
+     * default init); a component left uninitialised (FluidInit.None) keeps it */
     {
       size_t s;
-      for (s = 0; s < L->sites_h * q; s++) L->fluids[i][s] = NAN;
+      for (s = 0; s < L->sites_h * q; s++) L->fluids[i][s] = or_d_double_init();
     }
   }
   L->advection = (double *)malloc(sizeof(double) * L->sites_h * q);
   {
     size_t s;
-    for (s = 0; s < L->sites_h * q; s++) L->advection[s] = NAN;
+    for (s = 0; s < L->sites_h * q; s++) L->advection[s] = or_d_double_init();
   }
   L->mask = (int32_t *)calloc(L->sites_h, sizeof(int32_t));
   L->force_distributed = (double *)calloc(L->sites_h * d, sizeof(double));
@@ -210,6 +223,32 @@ int or_set_params(or_lattice *L, const double *tau, const double *gcc, const dou
   L->psi_form = psi_form;
   L->enable_sc = enable_shan_chen;
   return 0;
+}
+
+/* A body force as a plugin would add it: a preCollisionHooks callback
+ * (lb/collision.d:29, run at lb/collision.d:68-70 after distributeForce and
+ * prepareToCollide) doing L.force[f][p][k] += body[f][k] over the force
+ * field.  body[nc*d] row-major; NULL or all zero registers no hook. */
+int or_set_body_force(or_lattice *L, const double *body) {
+  int f, k;
+  L->has_body = 0;
+  for (f = 0; f < OR_MAXC; f++)
+    for (k = 0; k < 3; k++) {
+      L->body[f][k] = (body && f < L->nc && k < L->d) ? body[f * L->d + k] : 0.0;
+      if (L->body[f][k] != 0.0) L->has_body = 1;
+    }
+  return 0;
+}
+
+/* preCollisionHooks.execute(L) with the body-force hook above, every stored
+ * site (the hook iterates the whole force field) */
+void or_pre_collision_hooks(or_lattice *L) {
+  int f, k;
+  size_t s;
+  if (!L->has_body) return;
+  for (f = 0; f < L->nc; f++)
+    for (s = 0; s < L->sites_h; s++)
+      for (k = 0; k < L->d; k++) L->force[f][s * L->d + k] += L->body[f][k];
 }
 
 size_t or_sites_h(const or_lattice *L) { return L->sites_h; }
@@ -810,6 +849,7 @@ void or_run_timesteps(or_lattice *L, int nsteps) {
     or_add_shan_chen_force(L);
     or_distribute_force(L);
     or_prepare_to_collide(L);
+    or_pre_collision_hooks(L); /* collideFields runs them first, collision.d:68-70 */
     or_collide_fields(L);
   }
 }

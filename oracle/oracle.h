@@ -39,11 +39,19 @@ typedef struct or_lattice or_lattice;
 /* Lattice with connectivity DdQq (D3Q19, D2Q9, D1Q3), C components,
  * physical lengths n[0..d).  Returns NULL on unsupported input. */
 or_lattice *or_create(int d, int q, int ncomp, const int *n);
+/* D's default-initialised double (a NaN), the populations' value before any init */
+double or_d_double_init(void);
 void or_destroy(or_lattice *L);
 
 int or_set_params(or_lattice *L, const double *tau, const double *gcc,
                   const double *gwc, const double *global_acc, int eq_dist_form,
                   int psi_form, int enable_shan_chen);
+
+/* Body force of a preCollisionHooks plugin (lb/collision.d:29,68-70):
+ * F_f += body[f*d + k] after distributeForce, before the collision.  NULL or
+ * all zero: no hook. */
+int or_set_body_force(or_lattice *L, const double *body);
+void or_pre_collision_hooks(or_lattice *L);
 
 /* Raw storage (halo-padded, AoS).  Sizes: or_sites_h(L) sites. */
 size_t or_sites_h(const or_lattice *L);

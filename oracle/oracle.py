@@ -58,6 +58,7 @@ def lib():
             "or_create": (vp, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
             "or_destroy": (None, [vp]),
             "or_set_params": (C.c_int, [vp, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int]),
+            "or_set_body_force": (C.c_int, [vp, C.POINTER(C.c_double)]),
             "or_sites_h": (C.c_size_t, [vp]),
             "or_fluid": (C.POINTER(C.c_double), [vp, C.c_int]),
             "or_mask": (C.POINTER(C.c_int32), [vp]),
@@ -188,6 +189,15 @@ class Oracle:
                                  int(eq_dist_form), int(psi_form), int(bool(enable_shan_chen)))
         if rc != 0:
             raise ValueError("oracle: bad parameters")
+
+    def set_body_force(self, body):
+        """Register the body-force pre-collision hook: F_f += body[f] after
+        distributeForce (lb/collision.d:29,68-70).  None removes it."""
+        if body is None:
+            lib().or_set_body_force(self.h, None)
+            return
+        b = np.ascontiguousarray(np.asarray(body, np.float64).reshape(self.nc * self.d))
+        lib().or_set_body_force(self.h, b.ctypes.data_as(C.POINTER(C.c_double)))
 
     # -- passes (src/main.d:174-200) ------------------------------------------
     def exchange_halo(self):
@@ -400,6 +410,8 @@ class Oracle:
         o.set_params(p.get("tau", [1.0] * nc), p.get("gcc"), p.get("gwc"), p.get("globalAcc"),
                      EQ_FORMS[p.get("eqDistForm", "SecondOrder")],
                      PSI_FORMS[p.get("psiForm", "Linear")], p.get("enableShanChen", False))
+        if p.get("bodyForce") is not None:  # a preCollisionHooks plugin
+            o.set_body_force(p["bodyForce"])
         lib().or_seed_global_rng(p.get("seed", 0) & 0xFFFFFFFF)  # initRNG, rank 0
         o.init_mask(p.get("maskInit", "None"), p.get("maskAxis", "X"),
                     p.get("wallOffset", 0), p.get("dimensionHasWall"),

@@ -87,6 +87,104 @@ CASES = {
                                   randomReseed=[True, True], randomSeed=[42, 43]),
 }
 
+CHECKPOINT_LOWDIM = {
+    # technical/checkpoint-2d/checkpoint-2d.in.source
+    "checkpoint_2d": dict(d=2, q=9, gn=[64, 64], components=2, tau=[1.0, 1.0],
+                          eqDistForm="BDist2", fluidInit=["EqDistTwoSpheresFrac"] * 2,
+                          fluidDensities=[[0.7, 0.04], [0.04, 0.7]], initRadius=0.2,
+                          initOffset=[0.0, 0.0], initSeparation=[0.4, 0.0],
+                          interfaceThickness=5.0, seed=42, gcc=[[0.0, 3.6], [3.6, 0.0]],
+                          enableShanChen=True, psiForm="Exponential"),
+    # technical/checkpoint-1d/checkpoint-d1q3.in.source (BDist2 has no D1Q3 form:
+    # eqdist.d:269-273 returns the population itself)
+    "checkpoint_d1q3": dict(d=1, q=3, gn=[64], components=2, tau=[1.0, 1.0],
+                            eqDistForm="BDist2", fluidInit=["EqDistSphereFrac"] * 2,
+                            fluidDensities=[[0.7, 0.04], [0.04, 0.7]], initRadius=0.2,
+                            initOffset=[0.0], interfaceThickness=5.0, seed=42,
+                            gcc=[[0.0, 3.6], [3.6, 0.0]], enableShanChen=True,
+                            psiForm="Exponential"),
+}
+
+# lb/init-2d/<name>.in: D2Q9, two components, t = 0.  Shapes on the reference's
+# global coordinates, so one rank over the whole lattice writes what the four
+# ranks of the reference wrote; the random kinds keep one rank's 32x32 share.
+_SHAPE = dict(eqDistForm="BDist2", fluidDensities=[[1.0, 0.0], [0.0, 1.0]],
+              interfaceThickness=4.0)
+_RAND = dict(eqDistForm="BDist2", gn=[32, 32], fluidDensities=[[0.5], [1.0]],
+             randomReseed=[True, True], randomSeed=[42, 43])
+_MASK = dict(fluidInit=["None", "None"], fluidDensities=[[], []])
+INIT_2D = {
+    "fluid_init_const_2d": dict(fluidInit=["Const"] * 2, fluidDensities=[[0.5], [1.0]]),
+    "fluid_init_constrandom_2d": dict(_RAND, fluidInit=["ConstRandom"] * 2, eqDistForm="SecondOrder"),
+    "fluid_init_eqdist_2d": dict(eqDistForm="BDist2", fluidInit=["EqDist"] * 2,
+                                 fluidDensities=[[0.5], [1.0]]),
+    "fluid_init_eqdistrandom_2d": dict(_RAND, fluidInit=["EqDistRandom"] * 2),
+    "fluid_init_eqdistperturb_2d": dict(_RAND, fluidInit=["EqDistPerturb"] * 2,
+                                        fluidPerturb=[0.1, 0.2]),
+    "fluid_init_eqdistperturbfrac_2d": dict(_RAND, fluidInit=["EqDistPerturbFrac"] * 2,
+                                            fluidPerturb=[0.2, 0.2]),
+    "fluid_init_eqdistsphere_2d": dict(_SHAPE, gn=[128, 64], fluidInit=["EqDistSphere"] * 2,
+                                       initRadius=16.0, initOffset=[12.8, 0.0]),
+    "fluid_init_eqdistspherefrac_2d": dict(_SHAPE, gn=[128, 64],
+                                           fluidInit=["EqDistSphereFrac"] * 2,
+                                           initRadius=0.25, initOffset=[0.1, 0.0]),
+    "fluid_init_eqdistspherefrac_boundary_2d": dict(_SHAPE, gn=[128, 64],
+                                                    fluidInit=["EqDistSphereFrac"] * 2,
+                                                    initRadius=0.25, initOffset=[0.5, 0.5]),
+    "fluid_init_eqdisttwospheres_2d": dict(_SHAPE, gn=[128, 64],
+                                           fluidInit=["EqDistTwoSpheres"] * 2, initRadius=16.0,
+                                           initOffset=[12.8, 0.0], initSeparation=[6.4, 0.0]),
+    "fluid_init_eqdisttwospheresfrac_2d": dict(_SHAPE, gn=[128, 64],
+                                               fluidInit=["EqDistTwoSpheresFrac"] * 2,
+                                               initRadius=0.25, initOffset=[0.1, 0.0],
+                                               initSeparation=[0.05, 0.0]),
+    "fluid_init_none_2d": dict(_MASK),
+    "mask_init_none_2d": dict(_MASK),
+    "mask_init_box_2d": dict(_MASK, maskInit="Box"),
+    "mask_init_walls_2d": dict(_MASK, maskInit="Walls",
+                               dimensionHasWall=[[True, False], [False, True]], wallThickness=2),
+}
+for _ax in "XYZ":
+    INIT_2D["fluid_init_eqdistcylinder_%s_2d" % _ax.lower()] = dict(
+        _SHAPE, gn=[128, 64], fluidInit=["EqDistCylinder"] * 2, initRadius=16.0,
+        initOffset=[12.8, 0.0], initAxis=_ax)
+    # lb/init-2d/fluid-init-eqdistcylinderfrac-y-2d.in takes 0.125 (x is the shorter
+    # axis other than y), the others 0.25
+    INIT_2D["fluid_init_eqdistcylinderfrac_%s_2d" % _ax.lower()] = dict(
+        _SHAPE, gn=[128, 64], fluidInit=["EqDistCylinderFrac"] * 2,
+        initRadius=0.125 if _ax == "Y" else 0.25, initOffset=[0.1, 0.0], initAxis=_ax)
+    INIT_2D["mask_init_tube_%s_2d" % _ax.lower()] = dict(_MASK, maskInit="Tube", maskAxis=_ax)
+for _ax in "XY":
+    INIT_2D["fluid_init_eqdistlamellae_%s_2d" % _ax.lower()] = dict(
+        eqDistForm="BDist2", gn=[128, 64], fluidInit=["EqDistLamellae"] * 2,
+        interfaceThickness=4.0, initAxis=_ax,
+        **({"lamellaeWidths": [64.0, 32.0, 16.0, 8.0],
+            "fluidDensities": [[1.0, 0.0, 1.0, 0.0], [1.0, 1.0, 0.0, 1.0]]} if _ax == "X" else
+           {"lamellaeWidths": [32.0, 16.0, 16.0],
+            "fluidDensities": [[1.0, 0.0, 1.0], [1.0, 1.0, 0.0]]}))
+    INIT_2D["fluid_init_eqdistlamellaefrac_%s_2d" % _ax.lower()] = dict(
+        eqDistForm="BDist2", gn=[128, 64], fluidInit=["EqDistLamellaeFrac"] * 2,
+        interfaceThickness=4.0, initAxis=_ax,
+        **({"lamellaeWidths": [0.5, 0.25, 0.125, 0.0625],
+            "fluidDensities": [[1.0, 0.0, 1.0, 0.0], [1.0, 1.0, 0.0, 1.0]]} if _ax == "X" else
+           {"lamellaeWidths": [0.5, 0.25, 0.25],
+            "fluidDensities": [[1.0, 0.0, 1.0], [1.0, 1.0, 0.0]]}))
+    INIT_2D["mask_init_plates_%s_2d" % _ax.lower()] = dict(_MASK, maskInit="Plates",
+                                                           maskAxis=_ax, wallOffset=4)
+# MaskInit.File and FluidInit.File read the very arrays the reference wrote out
+# (mask-init-file-2d.in, fluid-init-file-2d.in), so these two take their input
+# from the fixture itself.
+INIT_2D["mask_init_file_2d"] = dict(_MASK)
+INIT_2D["fluid_init_file_2d"] = dict(eqDistForm="BDist2", gn=[128, 64],
+                                     fluidInit=["File"] * 2, fluidDensities=[[], []])
+
+
+def init_2d_params(name):
+    """Parameters of one lb/init-2d case (gn defaults to the 64x64 lattice)."""
+    p = dict(d=2, q=9, gn=[64, 64], components=2, tau=[1.0, 1.0])
+    p.update(INIT_2D[name])
+    return p
+
 
 def case(name, **over):
     p = dict(CASES[name])

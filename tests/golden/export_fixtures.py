@@ -153,6 +153,70 @@ def export_lb3d(name, test, files, steps, nspatial, tol=None):
                         "shapes": shapes, "reduced_axes": reduced, "tolerance": tol})
 
 
+# Every lb/init-2d case (tests/runnable/tests/lb/init-2d/*.json): one fluid or
+# mask initialisation kind each, written by 4 ranks (nc = [2, 2]) at t = 0.
+# The random kinds seed every rank identically (shiftSeedByRank = false), so
+# their four quadrants are identical and one is kept.
+INIT_2D_RANDOM = ("fluid-init-constrandom-2d", "fluid-init-eqdistrandom-2d",
+                  "fluid-init-eqdistperturbfrac-2d", "fluid-init-eqdistperturb-2d")
+
+
+def export_init_2d():
+    d = os.path.join(REF, "lb/init-2d")
+    names = sorted(f[:-5] for f in os.listdir(d) if f.endswith(".json"))
+    arrays, meta = {}, {}
+    for name in names:
+        key = name.replace("-", "_")
+        fields = {}
+        for fld, prefix in (("population_red", "population-red"),
+                            ("population_blue", "population-blue"),
+                            ("colour", "colour-red-blue"), ("mask", "mask")):
+            fields[fld] = read_outarray(os.path.join(
+                d, "reference-data", "%s-%s-t00000000.h5" % (prefix, name)))
+        shape = list(fields["mask"].shape)
+        quadrant = name in INIT_2D_RANDOM
+        for fld, a in fields.items():
+            if quadrant:
+                h0, h1 = a.shape[0] // 2, a.shape[1] // 2
+                q = [a[:h0, :h1], a[h0:, :h1], a[:h0, h1:], a[h0:, h1:]]
+                bits = [x.view(np.uint64 if x.dtype.itemsize == 8 else np.uint32) for x in q]
+                assert all(np.array_equal(bits[0], b) for b in bits[1:]), (name, fld)
+                a = q[0].copy()
+            arrays["%s__%s" % (key, fld)] = a
+        meta[key] = {"params": "lb/init-2d/%s.in" % name, "shape": shape,
+                     "quadrant": quadrant}
+    save("init_2d", arrays,
+         {"source": "tests/runnable/tests/lb/init-2d/reference-data",
+          "layout": "np=4, nc=[2,2]; random kinds (shiftSeedByRank=false) keep one "
+                    "quadrant, the share of one rank", "cases": meta,
+          "tolerance": {"t0": 0.0}})
+
+
+def export_checkpoint_lowdim():
+    """technical/checkpoint-2d (D2Q9) and checkpoint-1d (D1Q3): restart state at
+    t = 0 plus the outputs at t = 0 and t = 10 (strict h5diff)."""
+    for name, sub, label in (("checkpoint_2d", "technical/checkpoint-2d", "checkpoint-2d"),
+                             ("checkpoint_d1q3", "technical/checkpoint-1d", "checkpoint-d1q3")):
+        d = os.path.join(REF, sub, "reference-data")
+        arrays = {}
+        for f, col in ((0, "red"), (1, "blue")):
+            cp = find(os.path.join(d, "cp"), r"cp-fluids%d-%s-.*t00000000\.h5" % (f, re.escape(label)))
+            assert read_timestep_attr(cp) == 0
+            arrays["cp_fluids%d_t0" % f] = read_outarray(cp)
+            for t in (0, 10):
+                for fld in ("population", "density"):
+                    arrays["%s_%s_t%d" % (fld, col, t)] = read_outarray(
+                        find(d, r"%s-%s-%s-.*t%08d\.h5" % (fld, col, re.escape(label), t)))
+        for t in (0, 10):
+            arrays["colour_t%d" % t] = read_outarray(
+                find(d, r"colour-red-blue-%s-.*t%08d\.h5" % (re.escape(label), t)))
+        arrays["cp_mask_t0"] = read_outarray(
+            find(os.path.join(d, "cp"), r"cp-mask-%s-.*t00000000\.h5" % re.escape(label)))
+        src = "%s/%s.in.source" % (sub, label)
+        save(name, arrays, {"source": "tests/runnable/tests/%s/reference-data" % sub,
+                            "tolerance": {"t0": 0.0, "t10": 0.0}, "params": src})
+
+
 def main():
     if not os.path.isdir(REF):
         sys.exit("reference tree not present")
@@ -251,6 +315,8 @@ def main():
           "params": "lb/init-2d/fluid-init-eqdistperturb-2d.in",
           "layout": "np=4, nc=[2,2], shiftSeedByRank=false: one 32x32 quadrant kept",
           "tolerance": {"t0": 0.0}})
+    export_init_2d()
+    export_checkpoint_lowdim()
     # Laplace diagnostics written every 100 steps (lb/laplace.d:55-281), printed
     # with 7 significant digits: t, sigma, measuredR, inPres, outPres, D.
     for dim in ("2d", "3d"):

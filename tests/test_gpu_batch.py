@@ -166,3 +166,76 @@ def test_shifted_split_schedule_equals_tiled(D, storage, walls, n):
     b.step(25)
     for f in range(2):
         assert np.array_equal(a.population(f).view(np.uint64), b.population(f).view(np.uint64))
+
+
+def test_config5_ensemble_16_lattices(D):
+    """The batched-ensemble workload cut to its first 16 lattices (L from 32
+    to 96, tau, g and radius per lattice from the workload's seed), FP32
+    shifted storage, 50 steps: the autotuned batch replayed from its graph
+    equals each lattice stepped alone and the per-lattice loop bit for bit,
+    and every lattice stays within the FP32 bounds of its own FP64 run
+    (largest density and velocity differences; measured on MI355X about a
+    third of the bounds)."""
+    import bench
+    params = bench.ensemble_params(16)
+    batch = [D.Lattice.from_params(p, storage="f32_shift") for p in params]
+    B = D.Batch(batch)
+    cfg, _ = B.autotune()
+    B.step(50, graph=True)
+    B.synchronize()
+    got = _pops(batch)
+    alone = [D.Lattice.from_params(p, storage="f32_shift") for p in params]
+    for L in alone:
+        L.step(50)
+    _same(got, _pops(alone), "graph batch vs alone (config %s)" % (cfg,))
+    looped = [D.Lattice.from_params(p, storage="f32_shift") for p in params]
+    C = D.Batch(looped)
+    C.set_config(*cfg)
+    C.step_looped(50)
+    C.synchronize()
+    _same(got, _pops(looped), "graph batch vs looped")
+    worst = {"drho": 0.0, "du": 0.0}
+    for k, p in enumerate(params):
+        ref = D.Lattice.from_params(p)
+        ref.step(50)
+        for f in range(2):
+            worst["drho"] = max(worst["drho"],
+                                float(np.abs(batch[k].density(f) - ref.density(f)).max()))
+            worst["du"] = max(worst["du"],
+                              float(np.abs(batch[k].velocity(f) - ref.velocity(f)).max()))
+        ref.close()
+    print("config5 16 lattices, 50 steps, FP32 vs FP64: %s (autotuned %s)" % (worst, cfg))
+    assert worst["drho"] <= 3e-6 and worst["du"] <= 3e-6, worst
+
+
+@pytest.mark.parametrize("kernel", [1, 2], ids=["tiled-no-solid", "split"])
+def test_batch_graph_after_member_gains_solids(D, kernel):
+    """The captured batch graph is dropped when a member gains solid sites (the
+    tiled kernel without bounce-back was captured) or when Shan-Chen is turned
+    on for all members: graph replays keep matching each lattice stepped
+    alone through the same sequence."""
+    batch, alone = _members(D), _members(D)
+    B = D.Batch(batch)
+    B.set_config(8, 1 << 20, 512, kernel)
+    B.step(4, graph=True)
+    for L in alone:
+        L.step(4)
+    _same(_pops(batch), _pops(alone), "before walls")
+    walls = [[False, False], [False, False], [True, True]]
+    for L in (batch[2], alone[2]):
+        L.init_mask("Walls", dim_has_wall=walls)
+        for f in range(2):
+            L.init_eq_dist_wall(f)
+    B.step(6, graph=True)
+    for L in alone:
+        L.step(6)
+    _same(_pops(batch), _pops(alone), "after walls on member 2")
+    # Shan-Chen off on every member (no psi pass), then on again
+    for on in (False, True):
+        for L in batch + alone:
+            L.set_params([1.1, 1.1], [[0.0, 3.9], [3.9, 0.0]], None, None, None, "BDist2",
+                         "Exponential", on)
+        B.step(4, graph=True)
+        for L in alone:
+            L.step(4)
+        _same(_pops(batch), _pops(alone), "Shan-Chen %s" % on)

@@ -186,3 +186,30 @@ def test_graph_after_restore(D, storage):
     a.step(70, graph=True)
     b.step(70)
     _same(a, b, "after restore")
+
+
+def test_mask_change_mid_run_matches_oracle(D, oracle_mod):
+    """Walls added after 17 steps (an odd count: the two population buffers
+    hold different states then): the new solid sites keep the populations
+    they had, as the reference's advection copies a solid site's populations
+    every step (advection.d:84-86).  Eager steps and graph replays both equal
+    the oracle run through the same sequence, bit for bit."""
+    p = _walls((16, 20, 24))
+    o = oracle_mod.Oracle.from_params(p)
+    a = D.Lattice.from_params(p)
+    b = D.Lattice.from_params(p)
+    a.step(17)
+    b.step(17, graph=True)
+    o.run(17)
+    walls = [[False, False], [True, True], [False, False]]
+    for L in (a, b):
+        L.init_mask("Walls", dim_has_wall=walls)
+    o.init_mask("Walls", 0, 0, walls, 1)
+    o.exchange_halo()
+    a.step(40)
+    b.step(40, graph=True)
+    o.run(40)
+    for L, what in ((a, "eager"), (b, "graph")):
+        for f in range(2):
+            assert np.array_equal(L.population(f).view(np.uint64),
+                                  o.population(f).view(np.uint64)), (what, f)

@@ -192,3 +192,40 @@ def test_refusals(D):
         L.init_eq_dist_wall(2)
     with pytest.raises(D.DlbcError):
         L.step(1)  # nothing initialised yet
+
+
+def _init_2d_names():
+    from cases import INIT_2D
+    return sorted(INIT_2D)
+
+
+@pytest.mark.parametrize("name", _init_2d_names())
+def test_init_2d_fixture_on_device(D, name):
+    """Every lb/init-2d case of the reference (each FluidInit and MaskInit
+    kind), initialised on the device and through the host initialisation plus
+    upload: populations (D's default NaN included), mask and colour equal the
+    reference's own t = 0 output bit for bit.  The File kinds take the
+    fixture as the file the reference reads."""
+    from cases import assert_close, golden, init_2d_params
+    g, _ = golden("init_2d")
+    p = init_2d_params(name)
+    if name == "mask_init_file_2d":
+        p["maskArray"] = g[name + "__mask"]
+    for device_init in (True, False):
+        L = D.Lattice.from_params(p, device_init=device_init)
+        if name == "fluid_init_file_2d":
+            for f, c in enumerate(("red", "blue")):
+                L.upload_population(f, g["%s__population_%s" % (name, c)])
+        what = "%s (%s)" % (name, "device" if device_init else "host + upload")
+        for f, c in enumerate(("red", "blue")):
+            assert_close(L.population(f), g["%s__population_%s" % (name, c)], 0.0,
+                         "%s %s" % (what, c))
+        assert np.array_equal(L.mask(), g[name + "__mask"]), what + " mask"
+        # colour is computed from the populations: where the reference's is NaN
+        # (uninitialised components) the GPU's must be NaN too, with whatever
+        # payload its arithmetic gives; elsewhere bit for bit
+        col, want = L.colour(), g[name + "__colour"]
+        nan = np.isnan(want)
+        assert np.array_equal(np.isnan(col), nan), what + " colour NaN sites"
+        assert_close(np.where(nan, 0.0, col), np.where(nan, 0.0, want), 0.0, what + " colour")
+        L.close()

@@ -152,13 +152,16 @@ def test_config2_spinodal_bitexact(D, oracle_mod, form):
     _same_state(g, o, "config2 " + form)
 
 
-@pytest.mark.skipif(not os.environ.get("DLBC_FULL_PARITY"),
-                    reason="about 70 s of oracle time per form; set DLBC_FULL_PARITY=1")
-@pytest.mark.parametrize("form", ["SecondOrder", "BDist2"])
+@pytest.mark.parametrize("form", [
+    "SecondOrder",
+    pytest.param("BDist2", marks=pytest.mark.skipif(
+        not os.environ.get("DLBC_FULL_PARITY"),
+        reason="a second 70 s of oracle time; set DLBC_FULL_PARITY=1"))])
 def test_config2_full_size_1000_steps_bitexact(D, oracle_mod, form):
-    """The 128^3 spinodal workload at its full size and length, 1000 steps:
-    bit-identical to the oracle.  (Run on MI355X: both forms equal; the GPU
-    took 0.4-0.6 s including initialisation, the oracle 67-78 s on 16 cores.)"""
+    """The 128^3 spinodal workload at its full size and length, 1000 steps,
+    with the reference's default SecondOrder equilibrium (BDist2 on request):
+    bit-identical to the oracle.  (On MI355X the GPU takes 0.4-0.6 s
+    including initialisation, the oracle 67-78 s on 16 cores.)"""
     p = _config2(128, form)
     g, o = _pair(D, oracle_mod, p)
     g.step(1000)
@@ -228,9 +231,9 @@ def test_quasi_1d_across_dimensions(D):
 
 
 def test_walls_wetting_bodyforce_bitexact(D, oracle_mod):
-    """Walls with a wall-fluid coupling (gwc) and random solid sites, which
-    no reference fixture exercises: GPU == oracle.  (The oracle has no body
-    force; it is tested separately below.)"""
+    """Walls with a wall-fluid coupling (gwc), random solid sites and a body
+    force on both components, which no reference fixture exercises: GPU ==
+    oracle."""
     rng = np.random.default_rng(7)
     n = [12, 10, 16]
     mask = (rng.random(n) < 0.1).astype(np.int32)
@@ -240,7 +243,8 @@ def test_walls_wetting_bodyforce_bitexact(D, oracle_mod):
              gwc=[0.3, -0.2], enableShanChen=True, psiForm="Exponential",
              globalAcc=[1e-5, 0.0, 0.0], fluidInit=["EqDistPerturb"] * 2,
              fluidDensities=[[0.7], [0.3]], fluidPerturb=[0.05, 0.05],
-             randomReseed=[True, True], randomSeed=[5, 6], maskArray=mask)
+             randomReseed=[True, True], randomSeed=[5, 6], maskArray=mask,
+             bodyForce=[[0.0, 3e-6, 0.0], [-2e-6, 0.0, 1e-6]])
     for form in ("SecondOrder", "BDist2"):
         p["eqDistForm"] = form
         g, o = _pair(D, oracle_mod, p)
@@ -583,16 +587,45 @@ def test_restart_from_hdf5_checkpoint(D, tmp_path):
 def test_bench_channel_workload_bitexact_at_64(D, oracle_mod):
     """The bench.py channel workload (walls at z = 0 and n - 1, seeded random
     solid spheres to a solid fraction of 0.2, red/blue lamellae along x,
-    BDist2, g 3.6) at 64^3 on the default schedule, 20 steps: bit-identical
-    to the oracle.  The body force is left out: the oracle has none (the
-    reference adds it from a plugin hook); it is checked on its own above."""
+    BDist2, g 3.6, body force 1e-6 along x on component 1) at 64^3 on the
+    default schedule, 20 steps: bit-identical to the oracle, whose body force
+    is the reference's pre-collision hook (lb/collision.d:29,68-70)."""
     import bench
     n = 64
     p = bench.channel_params(n)
-    p.pop("bodyForce")
+    assert p["bodyForce"][1][0] == 1e-6
     p["maskArray"] = bench.channel_mask(n)
     g, o = _pair(D, oracle_mod, p)
     assert g.schedule == "split"
     g.step(20)
     o.run(20)
     _same_state(g, o, "channel 64^3")
+
+
+@pytest.mark.parametrize("name", ["checkpoint_2d", "checkpoint_d1q3"])
+def test_checkpoint_lowdim_restart(D, oracle_mod, name):
+    """technical/checkpoint-2d (D2Q9) and checkpoint-1d (D1Q3): the t0 restart
+    state uploaded and advanced 10 steps equals the oracle bit for bit, and the
+    reference's t10 output (bit for bit for D1Q3; within 6 ulps for D2Q9,
+    whose data an older revision wrote with D's x87 exp)."""
+    from cases import CHECKPOINT_LOWDIM
+    g, _ = golden(name)
+    p = CHECKPOINT_LOWDIM[name]
+    L = D.Lattice(p["d"], p["q"], 2, p["gn"])
+    L.set_params(p["tau"], p.get("gcc"), None, None, None, p["eqDistForm"], p["psiForm"],
+                 p["enableShanChen"])
+    L.upload_mask(g["cp_mask_t0"])
+    o = oracle_mod.Oracle.from_params(p)
+    for f in range(2):
+        L.upload_population(f, g["cp_fluids%d_t0" % f])
+        o.set_population(f, g["cp_fluids%d_t0" % f])
+    o.set_mask(g["cp_mask_t0"])
+    o.exchange_halo()
+    L.step(10)
+    o.run(10)
+    _same_state(L, o, name)
+    tol = 2.5e-16 if name == "checkpoint_2d" else 0.0
+    for f, c in enumerate(("red", "blue")):
+        assert_close(L.population(f), g["population_%s_t10" % c], tol, "t10 " + c)
+        assert_close(L.density(f), g["density_%s_t10" % c], 2 * tol, "t10 density " + c)
+    assert_close(L.colour(), g["colour_t10"], 2 * tol, "t10 colour")

@@ -330,3 +330,117 @@ def test_quasi_1d_across_dimensions(oracle_mod):
             assert np.abs(prof[d][f] - prof[d][f][:, :1]).max() == 0.0
             assert np.abs(prof[d][f][:, 0] - prof[1][f][:, 0]).max() <= 1e-14
 
+
+
+@pytest.mark.parametrize("form", ["SecondOrder", "BDist2"])
+def test_body_force_momentum_closed_form(oracle_mod, form):
+    """The body-force pre-collision hook (F_f += b_f after distributeForce,
+    lb/collision.d:29,68-70) through the velocity shift u + tau F / rho: with
+    omega tau = 1 the collision gives the momentum j + F per site, so on a
+    periodic lattice each step adds exactly N b_f to the total momentum of
+    component f, on the forced component and axis only.  A force acting on
+    the wrong component, axis or with the wrong sign, or divided by rho
+    twice, fails here."""
+    n = [6, 5, 4]
+    N = 6 * 5 * 4
+    # BDist2 replaces the component's own velocity by the weighted velocity
+    # u_w = sum_f (j_f / tau_f) / sum_f rho_f (collision.d:156-196): with
+    # tau = 1 the components share momentum through it and their total grows
+    # by N (b_0 + b_1) per step; SecondOrder keeps each component's own
+    tau = [1.0, 1.0] if form == "BDist2" else [1.0, 0.8]
+    p = dict(d=3, q=19, gn=n, components=2, tau=tau, eqDistForm=form,
+             fluidInit=["EqDistPerturb"] * 2, fluidDensities=[[0.9], [0.4]],
+             fluidPerturb=[0.01, 0.01], randomReseed=[True, True], randomSeed=[3, 4],
+             bodyForce=[[0.0, 0.0, 0.0], [2e-5, -1e-5, 0.0]])
+    o = oracle_mod.Oracle.from_params(p)
+    j0 = [o.total_momentum(f) for f in range(2)]
+    steps = 7
+    o.run(steps)
+    j1 = [o.total_momentum(f) for f in range(2)]
+    want = np.array([2e-5, -1e-5, 0.0]) * N * steps
+    if form == "BDist2":
+        got = (j1[0] + j1[1]) - (j0[0] + j0[1])
+        assert np.allclose(got, want, rtol=0, atol=1e-15), got
+    else:
+        assert np.allclose(j1[1] - j0[1], want, rtol=0, atol=1e-15), j1[1] - j0[1]
+        assert np.allclose(j1[0] - j0[0], 0.0, rtol=0, atol=1e-15), j1[0] - j0[0]
+    # and without the hook nothing moves
+    q = oracle_mod.Oracle.from_params(dict(p, bodyForce=None))
+    q.run(steps)
+    drift = (q.total_momentum(0) + q.total_momentum(1)) - (j0[0] + j0[1])
+    assert np.allclose(drift, 0.0, rtol=0, atol=1e-15), drift
+
+
+def test_body_force_poiseuille_closed_form(oracle_mod):
+    """A body force density b on a component of density rho between plates
+    drives the same Poiseuille profile as the global acceleration b / rho:
+    v(x) = (b/rho) / (2 nu) (x - 1/2)(L - (x - 1/2)) at steady state, up to
+    the constant offset of the velocity-shift forcing (lb/poiseuille-2d, with
+    its globalAcc 1e-4 given as forces 0.7e-4 and 1e-4 on its two
+    components of density 0.7 and 1.0)."""
+    from cases import case
+    p = case("poiseuille_2d", globalAcc=[0.0, 0.0], bodyForce=[[0.0, 0.7e-4], [0.0, 1e-4]])
+    o = oracle_mod.Oracle.from_params(p)
+    o.run(30000)
+    x = np.arange(1, 31) - 0.5
+    nu = (1.0 - 0.5) / 3.0
+    an = 1e-4 / (2 * nu) * x * (30 - x)
+    for f in range(2):
+        v = o.velocity(f)[1:31, 0, 1]
+        off = v - an
+        assert np.ptp(off) < 1e-9, ("not the Poiseuille parabola", f, np.ptp(off))
+        assert abs(off[0]) < 2e-4
+        assert np.max(np.abs(o.velocity(f)[1:31, :, 0])) < 1e-12  # none across the plates
+
+
+@pytest.mark.parametrize("tau", [0.6, 0.8, 1.0, 1.25])
+def test_bdist2_total_momentum_factor(oracle_mod, tau):
+    """BDist2 relaxes every component towards the weighted velocity
+    u_w = sum_f (j_f / tau_f) / sum_f rho_f (lb/collision.d:156-196), and its
+    equilibrium carries rho_f u_w.  With equal tau the collision therefore
+    maps the total momentum J to (1 - omega) J + omega (omega J) =
+    (1 - omega + omega^2) J: conserved at tau = 1 only, amplified by a factor
+    above one for every tau < 1.  Checked on a uniform two-component state
+    moving at a small velocity, no forces, one step (streaming moves a
+    uniform state into itself)."""
+    n = [4, 4, 4]
+    o = oracle_mod.Oracle(3, 19, 2, n)
+    o.set_params([tau, tau], eq_dist_form=2)
+    vel, _, _, _ = oracle_mod.conn(3, 19)
+    rhos, u0 = (0.7, 0.3), np.array([1e-3, -2e-3, 5e-4])
+    for f, rho in enumerate(rhos):
+        # SecondOrder equilibrium at velocity u0 as the moving initial state
+        rest = np.zeros(19)
+        rest[0] = rho
+        eq = oracle_mod.eq_dist(3, 19, 0, rest, u0, rho)
+        o.set_population(f, np.broadcast_to(eq, tuple(n) + (19,)).copy())
+    o.exchange_halo()
+    j0 = o.total_momentum(0) + o.total_momentum(1)
+    o.run(1)
+    j1 = o.total_momentum(0) + o.total_momentum(1)
+    om = 1.0 / tau
+    assert np.allclose(j1, (1.0 - om + om * om) * j0, rtol=1e-12, atol=0), (j1, j0)
+
+
+@pytest.mark.parametrize("g", [3.0, 4.5])
+def test_ensemble_droplets_diverge_below_tau_one(oracle_mod, g):
+    """The batched-ensemble droplets (EqDistSphereFrac 0.7/0.04, radius 0.25 L,
+    BDist2) at 32^3: with tau = 0.6, the low end of the ensemble's specified
+    range, the state is no longer finite within 500 steps for both ends of
+    the coupling range (the momentum factor of the test above); with tau = 1
+    the same droplet stays finite.  This is why the ensemble draws tau from
+    [1.0, 1.5]."""
+    import bench
+    base = dict(bench.ensemble_params(1)[0], gn=[32, 32, 32], initRadius=0.25,
+                gcc=[[0.0, g], [g, 0.0]])
+    o = oracle_mod.Oracle.from_params(dict(base, tau=[0.6, 0.6]))
+    diverged_at = None
+    for t in range(25, 501, 25):
+        o.run(25)
+        if not np.all(np.isfinite(o.density(0))):
+            diverged_at = t
+            break
+    assert diverged_at is not None and diverged_at <= 500
+    s = oracle_mod.Oracle.from_params(dict(base, tau=[1.0, 1.0]))
+    s.run(200)
+    assert np.all(np.isfinite(s.density(0))) and np.all(np.isfinite(s.density(1)))
