@@ -154,7 +154,8 @@ def storage_leg(D, p, mask, pops, storage, steps, warmup, fluid):
     sites = int(np.prod(p["gn"]))
     el, kern = L.step_timed(max(5, steps // 2))
     dom = max(kern, key=kern.get)
-    b = alg_bytes(fluid, sites, {"f32_shift": 4, "f16_shift": 2}[storage])
+    elem = {"f32_shift": 4, "f16_shift": 2}[storage]
+    b = alg_bytes(fluid, sites, elem)
     drift = max(abs(L.total_mass(f) - m0[f]) / m0[f] for f in range(2))
     out = {"value": round(sites * steps / dt / 1e6, 2), "unit": "MLUPS",
            "ms_per_step": round(1e3 * dt / steps, 4), "kernel": dom,
@@ -162,6 +163,7 @@ def storage_leg(D, p, mask, pops, storage, steps, warmup, fluid):
            "achieved": round(b / (kern[dom] * 1e-3) / 1e9, 1),
            "frac": round(b / (kern[dom] * 1e-3) / HBM_PEAK, 4),
            "traffic": pmc_traffic(storage, dom, p["gn"][0]), "rho0": rho0,
+           "step_frac_per_site": round(sites * steps / dt * (2 * C * Q * elem + 1) / HBM_PEAK, 4),
            "mass_rel_drift": float("%.3g" % drift)}
     del L
     return out
@@ -603,6 +605,9 @@ def main():
                      "bytes_per_launch": b_alg, "launch_ms": round(kern[dom], 4),
                      "step_frac": round(mlups * 1e6 * (alg_bytes(fluid, sites, 8) / sites)
                                         / HBM_PEAK, 4),
+                     # the metric's own "% HBM peak": MLUPS x 609 B per site
+                     # (2 C Q 8 B + the mask byte, every site counted)
+                     "step_frac_per_site": round(mlups * 1e6 * (2 * C * Q * 8 + 1) / HBM_PEAK, 4),
                      "kernels": per_kernel},
     }
     del L

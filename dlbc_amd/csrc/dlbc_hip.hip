@@ -289,15 +289,39 @@ __global__ void k_box_density(Ld ld, const uint8_t *__restrict__ mask, double *_
 }
 
 // Deterministic double-double reduction over fluid sites.  Stage 1: every
-// block sums a fixed contiguous chunk (thread-sequential then tree); stage 2:
-// one block sums the block partials the same way.  what: 0 mass, 1..3 momentum.
+// block sums a fixed contiguous chunk (thread-sequential, then across each
+// wavefront by butterfly shuffles, then the block's four wave sums in order);
+// stage 2: one block sums the block partials the same way.  what: 0 mass,
+// 1..3 momentum.  The error-free additions are symmetric, so every lane of a
+// butterfly ends with the same sum and the result does not depend on timing.
 __device__ inline DD dd_plus(DD a, DD b) { return dd_add(a, b); }
+
+__device__ inline DD wave_dd_sum(DD v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1)
+    v = dd_plus(v, DD{__shfl_xor(v.hi, m, 64), __shfl_xor(v.lo, m, 64)});
+  return v;
+}
+
+// Sum of the 256 threads' values: lane sums per wave, then waves 0..3 in order.
+__device__ inline DD block_dd_sum(DD v) {
+  __shared__ double wh[4], wl[4];
+  v = wave_dd_sum(v);
+  const int t = threadIdx.x;
+  if ((t & 63) == 0) {
+    wh[t >> 6] = v.hi;
+    wl[t >> 6] = v.lo;
+  }
+  __syncthreads();
+  DD r{wh[0], wl[0]};
+  for (int w = 1; w < 4; ++w) r = dd_plus(r, DD{wh[w], wl[w]});
+  return r;
+}
 
 template <class L, class Ld>
 __global__ void __launch_bounds__(256) k_reduce1(Ld ld, const uint8_t *__restrict__ mask,
                                                  double *__restrict__ part, Geo g, int f,
                                                  int what, int chunk, int has_solid) {
-  __shared__ double sh[256], sl[256];
   const int b = blockIdx.x, t = threadIdx.x;
   const int begin = b * chunk;
   const int end = min(begin + chunk, g.sites);
@@ -317,43 +341,22 @@ __global__ void __launch_bounds__(256) k_reduce1(Ld ld, const uint8_t *__restric
     }
     acc = dd_plus(acc, DD{v, 0.0});
   }
-  sh[t] = acc.hi;
-  sl[t] = acc.lo;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (t < w) {
-      const DD r = dd_plus(DD{sh[t], sl[t]}, DD{sh[t + w], sl[t + w]});
-      sh[t] = r.hi;
-      sl[t] = r.lo;
-    }
-    __syncthreads();
-  }
+  const DD r = block_dd_sum(acc);
   if (t == 0) {
-    part[2 * b] = sh[0];
-    part[2 * b + 1] = sl[0];
+    part[2 * b] = r.hi;
+    part[2 * b + 1] = r.lo;
   }
 }
 
 __global__ void __launch_bounds__(256) k_reduce2(const double *__restrict__ part, int nparts,
                                                  double *__restrict__ out) {
-  __shared__ double sh[256], sl[256];
   const int t = threadIdx.x;
   DD acc{0.0, 0.0};
   for (int i = t; i < nparts; i += 256) acc = dd_plus(acc, DD{part[2 * i], part[2 * i + 1]});
-  sh[t] = acc.hi;
-  sl[t] = acc.lo;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (t < w) {
-      const DD r = dd_plus(DD{sh[t], sl[t]}, DD{sh[t + w], sl[t + w]});
-      sh[t] = r.hi;
-      sl[t] = r.lo;
-    }
-    __syncthreads();
-  }
+  const DD r = block_dd_sum(acc);
   if (t == 0) {
-    out[0] = sh[0];
-    out[1] = sl[0];
+    out[0] = r.hi;
+    out[1] = r.lo;
   }
 }
 
@@ -687,12 +690,19 @@ int field_launch(dlbc_gpu_ctx *c, double *out, int kind, int f1, int f2) {
   return DLBC_OK;
 }
 
-int reduce_launch(dlbc_gpu_ctx *c, int f, int what, double *out2) {
-  const int chunk = 1 << 16;
+// Sites per block of the first reduction stage, and the doubles of device
+// memory a reduction needs: two per block partial and two for the result.
+constexpr int kReduceChunk = 1 << 16;
+inline size_t reduce_doubles(const dlbc_gpu_ctx *c) {
+  return 2 * (size_t)blocks_for(c->geo.sites, kReduceChunk) + 2;
+}
+
+// The two reduction stages into device memory `work` (reduce_doubles(c)
+// doubles), the result copied to out2 on the host.
+int reduce_into(dlbc_gpu_ctx *c, int f, int what, double *work, double *out2) {
+  const int chunk = kReduceChunk;
   const int nparts = blocks_for(c->geo.sites, chunk);
-  int rc = ensure_scratch(c, sizeof(double) * (2 * (size_t)nparts + 2));
-  if (rc) return rc;
-  double *part = c->scratch, *res = c->scratch + 2 * nparts;
+  double *part = work, *res = work + 2 * nparts;
   with_loader(c, [&](auto lat, auto ld) {
     hipLaunchKernelGGL((k_reduce1<decltype(lat), decltype(ld)>), dim3(nparts), dim3(256), 0,
                        c->stream, ld, c->mask, part, c->geo, f, what, chunk, c->has_solid);
@@ -703,6 +713,12 @@ int reduce_launch(dlbc_gpu_ctx *c, int f, int what, double *out2) {
   HIPCHK(c, hipMemcpyAsync(out2, res, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return DLBC_OK;
+}
+
+int reduce_launch(dlbc_gpu_ctx *c, int f, int what, double *out2) {
+  int rc = ensure_scratch(c, sizeof(double) * reduce_doubles(c));
+  if (rc) return rc;
+  return reduce_into(c, f, what, c->scratch, out2);
 }
 
 // volumeAveragedDensity (src/dlbc/lb/laplace.d:236-281) with volumeAverage 1:
