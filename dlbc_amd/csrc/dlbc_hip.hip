@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -721,6 +722,42 @@ int reduce_launch(dlbc_gpu_ctx *c, int f, int what, double *out2) {
   return reduce_into(c, f, what, c->scratch, out2);
 }
 
+// ---- the allocation-free mass reduction --------------------------------------
+// dlbc_gpu_total_mass_noalloc keeps its block partials in this file-scope
+// array instead of context scratch.  Every GPU holds its own copy, at its own
+// address, until the process ends; every call on a GPU shares it, so calls
+// that overlap on one GPU race.  In host code the array's name is a host-side
+// stand-in, never a device address: kernels get the address that
+// hipGetSymbolAddress returned for the device (noalloc_ptr).  Nothing in it
+// must start at zero: every block writes its own partial, the second stage
+// writes the result.
+constexpr int kNoAllocDoubles = 8192;  // 64 KiB: 4095 partials, 2^16 sites each
+static __device__ __align__(256) double noalloc_scratch[kNoAllocDoubles];
+
+// The largest lattice it serves: (8192 - 2) / 2 partials of kReduceChunk sites.
+constexpr int64_t kNoAllocMaxSites = (int64_t)((kNoAllocDoubles - 2) / 2) * kReduceChunk;
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex noalloc_mu;
+double *noalloc_ptr[kNoAllocMaxDevices] = {};
+
+// Look up noalloc_scratch's device address on c->device, once per device; the
+// caller has made that device current (dlbc_gpu_create, after hipSetDevice).
+int noalloc_setup(dlbc_gpu_ctx *c) {
+  if (c->device < 0 || c->device >= kNoAllocMaxDevices)
+    return fail(c, DLBC_ERR_UNSUPPORTED, "device %d: the reduction scratch covers devices 0..%d",
+                c->device, kNoAllocMaxDevices - 1);
+  std::lock_guard<std::mutex> lock(noalloc_mu);
+  if (noalloc_ptr[c->device]) return DLBC_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch));
+  if (e != hipSuccess || !p)
+    return fail(c, DLBC_ERR_HIP, "hipGetSymbolAddress(noalloc_scratch) failed on device %d: %s",
+                c->device, hipGetErrorString(e));
+  noalloc_ptr[c->device] = static_cast<double *>(p);
+  return DLBC_OK;
+}
+
 // volumeAveragedDensity (src/dlbc/lb/laplace.d:236-281) with volumeAverage 1:
 // mean density per component over the fluid sites of the 3^d cube centred at
 // `centre` (clipped at the lattice edge, as the reference's cube does not
@@ -855,6 +892,7 @@ int dlbc_gpu_create(dlbc_gpu_ctx **out, int d, int q, int components, const int6
     return bail(DLBC_ERR_UNSUPPORTED);  // the tiled kernel needs n >= 4 on every axis
   c->elem = storage == DLBC_STORAGE_F64 ? 8 : (storage == DLBC_STORAGE_F32_SHIFT ? 4 : 2);
   if (hipSetDevice(device) != hipSuccess) return bail(DLBC_ERR_HIP);
+  if (const int rc = noalloc_setup(c)) return bail(rc);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(DLBC_ERR_HIP);
   const size_t popb = (size_t)c->elem * components * q * c->geo.sp;
@@ -1441,6 +1479,24 @@ int dlbc_gpu_colour(dlbc_gpu_ctx *c, int f1, int f2, double *host, int halo) {
 int dlbc_gpu_total_mass(dlbc_gpu_ctx *c, int f, double *out) {
   if (!c || !out || f < 0 || f >= c->C) return DLBC_ERR_ARG;
   return reduce_launch(c, f, 0, out);
+}
+
+int dlbc_gpu_total_mass_noalloc(dlbc_gpu_ctx *c, int f, double *out) {
+  if (!c || !out || f < 0 || f >= c->C) return DLBC_ERR_ARG;
+  if ((int64_t)c->geo.sites > kNoAllocMaxSites)
+    return fail(c, DLBC_ERR_UNSUPPORTED,
+                "total_mass_noalloc: %d sites exceed its fixed scratch (%d doubles, at most "
+                "%lld sites); use dlbc_gpu_total_mass",
+                c->geo.sites, kNoAllocDoubles, (long long)kNoAllocMaxSites);
+  double *work = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    if (c->device >= 0 && c->device < kNoAllocMaxDevices) work = noalloc_ptr[c->device];
+  }
+  if (!work)
+    return fail(c, DLBC_ERR_STATE, "total_mass_noalloc: no scratch address for device %d",
+                c->device);
+  return reduce_into(c, f, 0, work, out);
 }
 
 int dlbc_gpu_total_momentum(dlbc_gpu_ctx *c, int f, double *out) {

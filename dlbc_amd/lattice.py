@@ -81,6 +81,7 @@ def lib():
         "dlbc_gpu_velocity": (i, [vp, i, _dp, i]),
         "dlbc_gpu_colour": (i, [vp, i, i, _dp, i]),
         "dlbc_gpu_total_mass": (i, [vp, i, _dp]),
+        "dlbc_gpu_total_mass_noalloc": (i, [vp, i, _dp]),
         "dlbc_gpu_total_momentum": (i, [vp, i, _dp]),
         "dlbc_gpu_eval_psi": (i, [vp, i, _dp, _dp, i64]),
         "dlbc_gpu_last_error": (C.c_char_p, [vp]),

@@ -181,6 +181,18 @@ int dlbc_gpu_colour(dlbc_gpu_ctx *ctx, int f1, int f2, double *host, int halo);
 int dlbc_gpu_total_mass(dlbc_gpu_ctx *ctx, int f, double *out /*2*/);
 int dlbc_gpu_total_momentum(dlbc_gpu_ctx *ctx, int f, double *out /*2d*/);
 
+/* dlbc_gpu_total_mass without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (stepping
+ * loops that read the mass every few steps, harnesses that forbid hipMalloc).
+ * Same kernels and the same bits as dlbc_gpu_total_mass, but the block
+ * partials live in one fixed 64 KiB array per GPU (held until the process
+ * ends) instead of context scratch, so it serves lattices of at most
+ * 4095 * 65536 = 268,369,920 sites and returns DLBC_ERR_UNSUPPORTED above
+ * that.  Every context on a GPU shares that GPU's array: calls that overlap
+ * on one GPU (from several threads or streams) race.  Like
+ * dlbc_gpu_total_mass it waits for the context's stream. */
+int dlbc_gpu_total_mass_noalloc(dlbc_gpu_ctx *ctx, int f, double *out /*2*/);
+
 /* Laplace diagnostics of a centred two-component droplet, as dumpLaplace
  * (src/dlbc/lb/laplace.d:55-218): out[0] sigma, out[1] measured radius,
  * out[2] pressure inside, out[3] pressure outside (volume-averaged densities
