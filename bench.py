@@ -32,6 +32,32 @@ HBM_PEAK = 8.0e12  # B/s, MI355X
 C, Q = 2, 19
 
 
+def shift_rho0(storage, light=0.05, dense=1.0):
+    """Reference density of the shifted storages for components with two bulk
+    phases (light, dense).  FP16: the harmonic mean 2 ab / (a + b), at which
+    the stored deviation |rho - rho0| is the same fraction of rho in both
+    phases (0.905 for 0.05 / 1.0): with 11 bits per deviation the light
+    phase's populations are what limits FP16, and the midpoint leaves their
+    deviations at 9.5 times their density (measured on the 128^3 droplet
+    after 1000 steps: density error 0.009 against 0.113, radius 0.4 %
+    against 2.9 %).  FP32: the midpoint (a + b) / 2, the smallest deviations
+    overall; its error is set by the dense phase's arithmetic (the 256^3
+    channel after 2000 steps: largest velocity error 2.4e-5 of the largest
+    speed with the midpoint, 1.1e-4 with the harmonic mean)."""
+    if storage == "f16_shift":
+        return 2.0 * light * dense / (light + dense)
+    return 0.5 * (light + dense)
+
+
+RHO0 = None  # --rho0: one reference density for every shifted leg (None: shift_rho0)
+
+
+def rho0_for(storage):
+    if RHO0 is not None:
+        return list(RHO0)
+    return [shift_rho0(storage)] * 2
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -102,6 +128,8 @@ def pmc_traffic(storage, kernel, n):
 
 
 BLOCK_ORDER = -1  # --block-order: split-schedule XCD block order (-1: the library's choice)
+MARCH = (0, -1)  # --march: rows per workgroup and hold of the march schedule
+LAUNCH = 0  # --launch: minimum workgroups per fused / march launch (0: default)
 
 
 def make_lattice(D, p, mask, pops, schedule, storage="f64", rho0=None):
@@ -110,6 +138,9 @@ def make_lattice(D, p, mask, pops, schedule, storage="f64", rho0=None):
                  True, rho0)
     if schedule != "auto":
         L.set_schedule(schedule)
+    if L.schedule == "march":
+        L.set_march(*MARCH)
+    L.set_launch(LAUNCH)
     L.set_block_order(BLOCK_ORDER)
     L.upload_mask(mask)
     for f in range(2):
@@ -147,7 +178,7 @@ def time_steps(L, steps, warmup):
 
 def storage_leg(D, p, mask, pops, storage, steps, warmup, fluid):
     """MLUPS and roofline of the same workload with shifted storage."""
-    rho0 = [0.525, 0.525]  # midpoint of the two bulk densities
+    rho0 = rho0_for(storage)
     L = make_lattice(D, p, mask, pops, "auto", storage, rho0)
     m0 = [L.total_mass(f) for f in range(2)]
     dt = time_steps(L, steps, warmup)
@@ -178,7 +209,7 @@ def storage_accuracy(D, p, mask, pops, storages, steps):
 
     def run(storage):
         L = make_lattice(D, p, mask, pops, "auto", storage,
-                         None if storage == "f64" else [0.525, 0.525])
+                         None if storage == "f64" else rho0_for(storage))
         m0 = [L.total_mass(f) for f in range(2)]
         L.step(steps)
         out = {"rho": [L.density(f) for f in range(2)],
@@ -203,7 +234,8 @@ def storage_accuracy(D, p, mask, pops, storages, steps):
             "max_abs_du_over_max_u": max(float(np.abs(d).max()) for d in du) / umax,
             "rel_l2_du": max(float(np.linalg.norm(du[f]) / np.linalg.norm(uref[f]))
                              for f in range(2)),
-            "mass_rel_drift": r["drift"], "f64_mass_rel_drift": ref["drift"]}
+            "mass_rel_drift": r["drift"], "f64_mass_rel_drift": ref["drift"],
+            "rho0": rho0_for(st)}
         log("accuracy %s after %d steps: %s" % (st, steps, res[st]))
     return res
 
@@ -279,7 +311,7 @@ def droplet_workload(D, n, steps, storages=("f16_shift", "f32_shift")):
     def run(storage):
         L = D.Lattice(3, Q, 2, p["gn"], storage=storage)
         L.set_params(p["tau"], p["gcc"], None, None, None, p["eqDistForm"], p["psiForm"], True,
-                     [0.525, 0.525])
+                     rho0_for(storage))
         if storage == "f64":
             # the host initialisation and upload, timed once beside the device one
             t = time.perf_counter()
@@ -354,7 +386,7 @@ def slab_workload(D, n, planes, steps, warmup):
     sites = planes * n * n
     out = {}
     for st in ("f64", "f32_shift", "f16_shift"):
-        L = make_lattice(D, ps, mask, pops, "auto", st, None if st == "f64" else [0.525, 0.525])
+        L = make_lattice(D, ps, mask, pops, "auto", st, None if st == "f64" else rho0_for(st))
         dt = time_steps(L, steps, warmup)
         el, kern = L.step_timed(max(5, steps // 2))
         elem = {"f64": 8, "f32_shift": 4, "f16_shift": 2}[st]
@@ -511,8 +543,18 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--schedule", default="auto", choices=["auto", "two_pass", "fused", "split"])
+    ap.add_argument("--schedule", default="auto",
+                    choices=["auto", "two_pass", "fused", "split", "march"])
+    ap.add_argument("--march", default="0,-1",
+                    help="march schedule launch shape: rows per workgroup, hold (0,-1: automatic)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--rho0", type=float, default=None,
+                    help="reference density of every shifted storage (default: FP16 the "
+                         "harmonic mean of the two bulk densities, %.4f; FP32 their midpoint, "
+                         "%.3f)" % (shift_rho0("f16_shift"), shift_rho0("f32_shift")))
+    ap.add_argument("--launch", type=int, default=0,
+                    help="minimum workgroups per launch of the fused and march kernels "
+                         "(0: the library's default)")
     ap.add_argument("--block-order", type=int, default=-1,
                     help="split schedule: blocks per XCD turn (0 site order, -1 automatic)")
     ap.add_argument("--no-storage-legs", action="store_true")
@@ -532,6 +574,12 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="time a plain FP64 streaming kernel of known size and exit")
     args = ap.parse_args()
+    global BLOCK_ORDER, MARCH, LAUNCH, RHO0
+    if args.rho0 is not None:
+        RHO0 = [args.rho0, args.rho0]
+    BLOCK_ORDER = args.block_order
+    LAUNCH = args.launch
+    MARCH = tuple(int(v) for v in args.march.split(","))
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
               file=sys.stderr)
@@ -555,8 +603,6 @@ def main():
     if args.workload == "small":
         small_workload(D)
         return
-    global BLOCK_ORDER
-    BLOCK_ORDER = args.block_order
     n = args.size
     p = channel_params(n)
     t0 = time.perf_counter()

@@ -15,6 +15,7 @@
 #include "kernels_batch.hpp"
 #include "kernels_fused.hpp"
 #include "kernels_init.hpp"
+#include "kernels_march.hpp"
 #include "kernels_shifted.hpp"
 #include "kernels_split.hpp"
 #include "kernels_twopass.hpp"
@@ -47,6 +48,8 @@ struct dlbc_gpu_ctx {
   StepParams p{};
   int schedule = DLBC_SCHED_AUTO;
   int target_blocks = 0;  // fused kernels: minimum workgroups per launch (0: default)
+  int march_rows = 0;     // march schedule: z-rows per workgroup (0: automatic)
+  int march_hold = -1;    // march schedule: populations held in registers (-1: automatic)
   // shifted storage: the tiled kernel over this lattice alone (a batch of one)
   void *bufX = nullptr, *bufY = nullptr;  // the two population buffers as allocated
   BatchLattice *d_self = nullptr;
@@ -504,6 +507,29 @@ struct FusedStep {
   }
 };
 
+// Single pass over whole z-rows (D3Q19, two components, FP64).
+template <class L, int C>
+struct MarchStep {
+  static int run(dlbc_gpu_ctx *c, Timer *tm, double *acc_ms) {
+    (void)acc_ms;
+    if constexpr (std::is_same<L, D3Q19>::value && C == 2) {
+      if (c->storage != DLBC_STORAGE_F64)
+        return fail(c, DLBC_ERR_UNSUPPORTED, "march schedule needs FP64 storage");
+      tm->before();
+      const int rc = launch_march_d3q19(static_cast<const double *>(c->popA),
+                                        static_cast<double *>(c->popB), c->nbw, c->S, c->geo,
+                                        c->p, c->stream, c->march_rows, c->march_hold,
+                                        c->target_blocks);
+      tm->after();
+      if (rc) return fail(c, DLBC_ERR_UNSUPPORTED, "march kernel: lattice shape not supported");
+      std::swap(c->popA, c->popB);
+      return DLBC_OK;
+    } else {
+      return fail(c, DLBC_ERR_UNSUPPORTED, "march schedule needs D3Q19 with 2 components");
+    }
+  }
+};
+
 // Lane-per-component two-pass schedule (D3Q19, two components, FP64).
 template <class L, int C>
 struct SplitStep {
@@ -607,6 +633,7 @@ int step_once(dlbc_gpu_ctx *c, int sched, Timer *tm) {
   tm->slot = 0;
   if (sched == DLBC_SCHED_FUSED) return dispatch<FusedStep>(c, tm, (double *)nullptr);
   if (sched == DLBC_SCHED_SPLIT) return dispatch<SplitStep>(c, tm, (double *)nullptr);
+  if (sched == DLBC_SCHED_MARCH) return dispatch<MarchStep>(c, tm, (double *)nullptr);
   return dispatch<TwoPassStep>(c, tm, (double *)nullptr);
 }
 
@@ -644,7 +671,8 @@ std::vector<unsigned char> graph_key(const dlbc_gpu_ctx *c, int sched) {
     const auto *b = static_cast<const unsigned char *>(p);
     k.insert(k.end(), b, b + n);
   };
-  const int ints[4] = {sched, c->storage, c->has_solid, c->target_blocks};
+  const int ints[6] = {sched,         c->storage,   c->has_solid,
+                       c->target_blocks, c->march_rows, c->march_hold};
   put(ints, sizeof ints);
   put(&c->geo, sizeof c->geo);
   put(&c->p, sizeof c->p);
@@ -1402,6 +1430,7 @@ const char *dlbc_gpu_kernel_name(const dlbc_gpu_ctx *c, int i) {
   if (sched == DLBC_SCHED_FUSED && c->storage != DLBC_STORAGE_F64)
     return i == 0 ? "tiled_d3q19_shifted" : "";
   if (sched == DLBC_SCHED_FUSED) return i == 0 ? "fused_d3q19" : "";
+  if (sched == DLBC_SCHED_MARCH) return i == 0 ? "march_d3q19" : "";
   if (sched == DLBC_SCHED_SPLIT) {
     const bool sh = c->storage != DLBC_STORAGE_F64;
     if (c->p.sc)
@@ -1422,7 +1451,17 @@ int dlbc_gpu_synchronize(dlbc_gpu_ctx *c) {
 }
 
 int dlbc_gpu_set_schedule(dlbc_gpu_ctx *c, int schedule) {
-  if (!c || schedule < 0 || schedule > 3) return DLBC_ERR_ARG;
+  if (!c || schedule < 0 || schedule > 4) return DLBC_ERR_ARG;
+  if (schedule == DLBC_SCHED_MARCH &&
+      !(c->d == 3 && c->C == 2 && c->storage == DLBC_STORAGE_F64 &&
+        [&] {
+          int ty;
+          bool h;
+          return march_shape(c->geo, c->march_rows, c->march_hold, ty, h);
+        }()))
+    return fail(c, DLBC_ERR_UNSUPPORTED,
+                "march schedule needs D3Q19, 2 components, FP64, n_z in {32, 64, 256, 512} "
+                "and, for n_z < 512, 28 sp < 2^29");
   if (schedule == DLBC_SCHED_SPLIT && !(c->d == 3 && c->C == 2))
     return fail(c, DLBC_ERR_UNSUPPORTED, "split schedule needs D3Q19 with 2 components");
   if (schedule == DLBC_SCHED_FUSED && c->storage == DLBC_STORAGE_F64 &&
@@ -1440,6 +1479,18 @@ int dlbc_gpu_get_schedule(const dlbc_gpu_ctx *c) { return c ? effective_schedule
 int dlbc_gpu_set_launch(dlbc_gpu_ctx *c, int min_workgroups) {
   if (!c || min_workgroups < 0 || min_workgroups > (1 << 20)) return DLBC_ERR_ARG;
   c->target_blocks = min_workgroups;
+  return DLBC_OK;
+}
+
+int dlbc_gpu_set_march(dlbc_gpu_ctx *c, int rows, int hold) {
+  if (!c || rows < 0 || rows > 2 || hold < -1 || hold > 1) return DLBC_ERR_ARG;
+  int ty;
+  bool h;
+  if (!march_shape(c->geo, rows, hold, ty, h))
+    return fail(c, DLBC_ERR_UNSUPPORTED, "march schedule: rows %d, hold %d not served for n_y = "
+                "%d, n_z = %d", rows, hold, c->geo.ny, c->geo.nz);
+  c->march_rows = rows;
+  c->march_hold = hold;
   return DLBC_OK;
 }
 

@@ -18,7 +18,7 @@ FLUID_INIT = {n: i for i, n in enumerate([
 MASK_INIT = {"None": 0, "File": 1, "Plates": 2, "Tube": 3, "Box": 4, "Walls": 5}
 AXIS = {"X": 0, "Y": 1, "Z": 2}
 STORAGE = {"f64": 0, "f32_shift": 1, "f16_shift": 2}
-SCHEDULES = {"auto": 0, "two_pass": 1, "fused": 2, "split": 3}
+SCHEDULES = {"auto": 0, "two_pass": 1, "fused": 2, "split": 3, "march": 4}
 ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "out of memory",
           -4: "unsupported configuration", -5: "call out of order"}
 
@@ -65,6 +65,7 @@ def lib():
         "dlbc_gpu_get_schedule": (i, [vp]),
         "dlbc_gpu_set_launch": (i, [vp, i]),
         "dlbc_gpu_set_block_order": (i, [vp, i]),
+        "dlbc_gpu_set_march": (i, [vp, i, i]),
         "dlbc_gpu_get_block_order": (i, [vp]),
         "dlbc_gpu_laplace": (i, [vp, _dp]),
         "dlbc_batch_create": (i, [C.POINTER(vp), C.POINTER(vp), i]),
@@ -322,6 +323,12 @@ class Lattice:
         """Minimum workgroups per fused-kernel launch (0: default)."""
         self._check(lib().dlbc_gpu_set_launch(self.h, int(min_workgroups)), "set_launch")
 
+    def set_march(self, rows=0, hold=-1):
+        """March schedule: z-rows per workgroup (1 or 2; 0 automatic) and
+        whether populations stay in registers between pull and collision
+        (1 yes, 0 pulled again, -1 automatic)."""
+        self._check(lib().dlbc_gpu_set_march(self.h, int(rows), int(hold)), "set_march")
+
     def set_block_order(self, chunk):
         """Split schedule: K > 0 blocks per XCD turn, 0 site order, -1 automatic."""
         self._check(lib().dlbc_gpu_set_block_order(self.h, int(chunk)), "set_block_order")
@@ -333,7 +340,7 @@ class Lattice:
     @property
     def schedule(self):
         v = lib().dlbc_gpu_get_schedule(self.h)
-        return {0: "auto", 1: "two_pass", 2: "fused", 3: "split"}[v]
+        return {0: "auto", 1: "two_pass", 2: "fused", 3: "split", 4: "march"}[v]
 
     def population(self, f, halo=0):
         out = np.zeros(_padded_shape(self.n, halo) + (self.q,))

@@ -64,7 +64,8 @@ enum {
   DLBC_SCHED_AUTO = 0,     /* fastest available for the configuration        */
   DLBC_SCHED_TWO_PASS = 1, /* psi pass + stream/force/collide pass (any DdQq)  */
   DLBC_SCHED_FUSED = 2,    /* single-pass D3Q19 kernel with an LDS psi ring   */
-  DLBC_SCHED_SPLIT = 3     /* D3Q19 two-component two-pass, lane per component */
+  DLBC_SCHED_SPLIT = 3,    /* D3Q19 two-component two-pass, lane per component */
+  DLBC_SCHED_MARCH = 4     /* D3Q19 two-component FP64 single pass over whole z-rows */
 };
 
 /* Create a context for one lattice on HIP device `device`.
@@ -159,6 +160,13 @@ int dlbc_gpu_set_launch(dlbc_gpu_ctx *ctx, int min_workgroups);
  * library choose.  The results do not depend on it.  No counterpart in the
  * reference. */
 int dlbc_gpu_set_block_order(dlbc_gpu_ctx *ctx, int chunk);
+/* Launch shape of the march schedule: rows = z-rows per workgroup (1, or 2
+ * for n_z <= 256 and n_y even; 0 the library's choice); hold = 1 keeps each
+ * site's pulled populations in registers from its psi pull to its collision
+ * (one row per workgroup), 0 pulls them again for the collision (-1 the
+ * library's choice).  Results do not depend on it.  No counterpart in the
+ * reference. */
+int dlbc_gpu_set_march(dlbc_gpu_ctx *ctx, int rows, int hold);
 int dlbc_gpu_get_block_order(const dlbc_gpu_ctx *ctx);
 
 /* Download component f into a host array [n0+2h]..[q]: physical sites are

@@ -432,6 +432,70 @@ def test_split_schedule_matches_oracle(D, oracle_mod, form, walls, n):
     _same_state(g, o, "split %s walls=%s" % (form, walls))
 
 
+# (z-rows per workgroup, populations held in registers, launch target):
+# one row pulling both neighbouring rows, the same holding its populations,
+# two rows, and the two row counts with one chunk of x per workgroup.
+MARCH_SHAPES = [(1, 0, 0), (1, 1, 0), (2, 0, 0), (1, 0, 1), (2, 0, 1)]
+
+
+@pytest.mark.parametrize("shape", MARCH_SHAPES, ids=["rows1", "rows1-hold", "rows2",
+                                                      "rows1-long", "rows2-long"])
+@pytest.mark.parametrize("form", ["SecondOrder", "BDist2", "ThirdOrder"])
+@pytest.mark.parametrize("walls", [False, True], ids=["periodic", "walls"])
+def test_march_schedule_matches_oracle(D, oracle_mod, form, walls, shape):
+    """The single-pass kernel over whole z-rows (LDS psi ring, x-marching)
+    against the oracle, bit for bit: chunks of one plane (the default target
+    on a small lattice) and one chunk per row tile, periodic and with walls,
+    random solids and gwc."""
+    n = [9, 6, 32]
+    p = _walls_case(n, form) if walls else dict(_config2(8, form), gn=n)
+    g = D.Lattice.from_params(p, schedule="march")
+    g.set_march(shape[0], shape[1])
+    if shape[2]:
+        g.set_launch(shape[2])
+    assert g.schedule == "march"
+    o = oracle_mod.Oracle.from_params(p)
+    g.step(40)
+    o.run(40)
+    _same_state(g, o, "march %s %s walls=%s" % (shape, form, walls))
+
+
+@pytest.mark.parametrize("n", [[7, 5, 64], [6, 4, 256], [3, 3, 512], [5, 2, 32]],
+                         ids=["nz64-odd-ny", "nz256", "nz512", "ny2"])
+def test_march_equals_split_with_body_force(D, n):
+    """The march schedule gives the split pair's bits on the other row
+    lengths it serves (64, 256 and 512 sites, n_y odd and n_y = 2), with
+    walls, random solids, gwc and the body force, in every launch shape."""
+    p = dict(_walls_case(n, "BDist2"), bodyForce=[[0.0, 0.0, 0.0], [1e-6, 0.0, 0.0]])
+    ref = D.Lattice.from_params(p, schedule="split")
+    ref.step(25)
+    want = [ref.population(f) for f in range(2)]
+    for rows, hold in [(0, -1), (1, 0), (1, 1), (2, 0)]:
+        g = D.Lattice.from_params(p, schedule="march")
+        if rows == 2 and (n[1] % 2 or n[2] > 256):
+            with pytest.raises(D.DlbcError):
+                g.set_march(rows, hold)
+            continue
+        g.set_march(rows, hold)
+        g.step(25)
+        for f in range(2):
+            assert_close(g.population(f), want[f], 0.0, "march %s %s f%d" % (n, (rows, hold), f))
+
+
+def test_march_refuses_unserved_lattices(D):
+    """Row lengths other than 32, 64, 256 and 512, shifted storage and
+    D2Q9 are refused with DLBC_ERR_UNSUPPORTED, and the automatic schedule
+    is unaffected."""
+    for n, storage in (([8, 8, 48], "f64"), ([8, 8, 128], "f64"), ([8, 8, 64], "f32_shift")):
+        L = D.Lattice(3, 19, 2, n, storage=storage)
+        with pytest.raises(D.DlbcError):
+            L.set_schedule("march")
+        assert L.schedule != "march"
+    L = D.Lattice(2, 9, 2, [32, 32])
+    with pytest.raises(D.DlbcError):
+        L.set_schedule("march")
+
+
 @pytest.mark.parametrize("n", [[10, 24, 40], [9, 10, 256]], ids=["gather", "staged"])
 @pytest.mark.parametrize("storage", ["f64", "f32_shift", "f16_shift"])
 def test_split_block_order_does_not_change_results(D, storage, n):
