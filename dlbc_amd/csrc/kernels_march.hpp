@@ -53,10 +53,6 @@ __host__ inline void march_chunks(const Geo &g, int ty, int target, int &lx, int
   nchunks = (g.nx + lx - 1) / lx;
 }
 
-// Wave-uniform copy of an int (the compiler then keeps it in a scalar
-// register and folds it into scalar address bases).
-__device__ inline int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
-
 // Element k of the array at a scalar base: with OFF32 the byte offset is
 // formed in 32 bits (k < 2^29), so the access takes the scalar base and a
 // 32-bit lane offset even where k differs between lanes.

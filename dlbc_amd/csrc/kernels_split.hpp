@@ -26,6 +26,8 @@
 // three give identical bits.
 #pragma once
 
+#include <type_traits>
+
 #include "kernels_fused.hpp"
 #include "kernels_shifted.hpp"
 
@@ -67,28 +69,37 @@ constexpr bool kSplitAutoChunk = false;
 // order_main for chunk K over a grid of nb blocks: whole turns of 8 K blocks.
 inline int split_order_main(int nb, int K) { return K > 0 ? nb / (8 * K) * (8 * K) : 0; }
 
+// Wave-uniform copy of an int: the compiler then keeps it in a scalar
+// register, and pointers formed from it stay scalar (the wave index
+// threadIdx.x >> 6 is uniform, but the compiler cannot tell).
+__device__ inline int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 // Site and component of this lane (the component is uniform per wave); false
 // past the end of the lattice.
 __device__ inline bool split_lane(const Geo &g, int &s, int &f) {
-  const int w = threadIdx.x >> 6;
+  const int w = uniform(threadIdx.x >> 6);
   f = w & 1;
-  s = split_block(g) * (kSplitThreads / 2) + (w >> 1) * 64 + (threadIdx.x & 63);
+  s = uniform(split_block(g) * (kSplitThreads / 2) + (w >> 1) * 64) + (threadIdx.x & 63);
   return s < g.sites;
 }
 
 // Element k of the array at `base` (wave-uniform).  OFF32 (every index of
-// the kernel below 2^29): the byte offset fits 32 bits, so the access takes a
-// scalar base and a 32-bit lane offset instead of a 64-bit lane address.
+// the kernel below 2^29): the byte offset is formed in 32 bits, so the access
+// takes the scalar base and a 32-bit lane offset instead of a 64-bit lane
+// address (one address register and no 64-bit arithmetic per access).
 template <bool OFF32, class T>
 __device__ inline T *elem(T *base, int64_t k) {
-  if constexpr (OFF32)
-    return reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (uint32_t)k * sizeof(T));
-  else
+  if constexpr (OFF32) {
+    using Byte = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+    return reinterpret_cast<T *>(reinterpret_cast<Byte *>(base) +
+                                 (uint32_t)((uint32_t)k * (uint32_t)sizeof(T)));
+  } else {
     return base + k;
+  }
 }
 template <bool OFF32, class T>
 __device__ inline const T &at(const T *base, int64_t k) {
-  return *elem<OFF32>(const_cast<T *>(base), k);
+  return *elem<OFF32>(base, k);
 }
 
 // Index offsets from a site to its neighbours along each axis with the
@@ -123,17 +134,24 @@ inline bool split_off32(const Geo &g) { return 19 * g.sp < ((int64_t)1 << 29); }
 
 // The 19 streamed populations of component f (array Af) at site s = (x, y, z)
 // whose solid word is sw (pull with bounce-back, src/dlbc/lb/advection.d:61-89).
+// Each load takes a wave-uniform base, array min(i, bounce(i)) of Af, and a
+// lane offset below 8 sp (bounce partners are at most 7 arrays apart).
+template <bool SOLID, bool OFF32, class T>
+__device__ inline T pull_one(const T *__restrict__ Af, const Geo &g, int i, int s,
+                             const NbrOffsets &o, uint32_t sw) {
+  using L = D3Q19;
+  const int src = nbr_site(s, o, -L::c[i][0], -L::c[i][1], -L::c[i][2]);
+  if (!SOLID) return at<OFF32>(Af + (int64_t)i * g.sp, src);
+  const int b = L::bounce[i], mc = i < b ? i : b;
+  const bool bb = (sw >> b) & 1u;
+  return at<OFF32>(Af + (int64_t)mc * g.sp, (int64_t)((bb ? b : i) - mc) * g.sp + (bb ? s : src));
+}
+
 template <bool SOLID, bool OFF32>
 __device__ inline void split_pull(const double *__restrict__ Af, const Geo &g, int s,
                                   const NbrOffsets &o, uint32_t sw, double *n) {
-  using L = D3Q19;
 #pragma unroll
-  for (int i = 0; i < L::q; ++i) {
-    const bool bb = SOLID && ((sw >> L::bounce[i]) & 1u);
-    const int src = bb ? s : nbr_site(s, o, -L::c[i][0], -L::c[i][1], -L::c[i][2]);
-    const int comp = bb ? L::bounce[i] : i;
-    n[i] = at<OFF32>(Af, (int64_t)comp * g.sp + src);
-  }
+  for (int i = 0; i < 19; ++i) n[i] = pull_one<SOLID, OFF32>(Af, g, i, s, o, sw);
 }
 
 // psi rows of a block of SB consecutive sites that lie in one z-row (n_z a
@@ -190,7 +208,7 @@ __global__ void __launch_bounds__(kSplitThreads)
   coords(s, g, x, y, z);
   double n[19];
   split_pull<SOLID, OFF32>(A + (int64_t)f * 19 * g.sp, g, s, nbr_offsets(g, x, y, z), sw, n);
-  *elem<OFF32>(S, (int64_t)f * g.sp + s) = psi_of(psi_form, density<D3Q19>(n));
+  *elem<OFF32>(S + (int64_t)f * g.sp, s) = psi_of(psi_form, density<D3Q19>(n));
 }
 
 // addShanChenForcePsi (src/dlbc/lb/force.d:238-309) for two components whose
@@ -379,12 +397,12 @@ __global__ void __launch_bounds__(kSplitThreads)
   double F[3];
   auto psi = [&](int ff, int i) {
     if (staged) return staged_psi<SB>(rows, ff, i, k);
-    return at<OFF32>(S, ff * g.sp + (i < 0 ? s : nbr(i)));
+    return at<OFF32>(S + ff * g.sp, i < 0 ? s : nbr(i));
   };
   auto solid = [&](int i) { return SOLID && ((sw >> i) & 1u); };
   auto rho_solid = [&](int ff, int i) {  // S holds the density at solid sites
     if (staged) return staged_psi<SB>(rows, ff, i, k);
-    return at<OFF32>(S, ff * g.sp + nbr(i));
+    return at<OFF32>(S + ff * g.sp, nbr(i));
   };
   if constexpr (OFFDIAG)
     sc_force_offdiag(f, F, p, psi, solid, rho_solid);
@@ -397,7 +415,7 @@ __global__ void __launch_bounds__(kSplitThreads)
                             p, wv);
   double *__restrict__ Bf = B + (int64_t)f * 19 * g.sp;
   collide_d3q19_streamed(f, n, rho, total, F, wv, p, [&](int i, double v) {
-    __builtin_nontemporal_store(v, elem<OFF32>(Bf, (int64_t)i * g.sp + s));
+    __builtin_nontemporal_store(v, elem<OFF32>(Bf + (int64_t)i * g.sp, s));
   });
 }
 
@@ -410,14 +428,8 @@ __global__ void __launch_bounds__(kSplitThreads)
 template <class Store, bool SOLID, bool OFF32>
 __device__ inline void ssplit_pull(const Store *__restrict__ Af, const Geo &g, int s,
                                    const NbrOffsets &o, uint32_t sw, float *n) {
-  using L = D3Q19;
 #pragma unroll
-  for (int i = 0; i < L::q; ++i) {
-    const bool bb = SOLID && ((sw >> L::bounce[i]) & 1u);
-    const int src = bb ? s : nbr_site(s, o, -L::c[i][0], -L::c[i][1], -L::c[i][2]);
-    const int comp = bb ? L::bounce[i] : i;
-    n[i] = to_f(at<OFF32>(Af, (int64_t)comp * g.sp + src));
-  }
+  for (int i = 0; i < 19; ++i) n[i] = to_f(pull_one<SOLID, OFF32>(Af, g, i, s, o, sw));
 }
 
 // Pass 1 for block blk of a lattice (the sites blk * 128 ..).
@@ -426,9 +438,9 @@ __device__ inline void ssplit_psi_block(const Store *__restrict__ A,
                                         const uint32_t *__restrict__ nbw,
                                         float *__restrict__ P, const Geo &g,
                                         const ShiftParams &p, int blk) {
-  const int w = threadIdx.x >> 6;
+  const int w = uniform(threadIdx.x >> 6);
   const int f = w & 1;
-  const int s = blk * (kShiftThreads / 2) + (w >> 1) * 64 + (threadIdx.x & 63);
+  const int s = uniform(blk * (kShiftThreads / 2) + (w >> 1) * 64) + (threadIdx.x & 63);
   if (s >= g.sites) return;
   const uint32_t sw = SOLID ? nbw[s] : 0u;
   if (sw >> 31) return;  // solid: psi is not read there
@@ -438,7 +450,7 @@ __device__ inline void ssplit_psi_block(const Store *__restrict__ A,
   ssplit_pull<Store, SOLID, OFF32>(A + (int64_t)f * 19 * g.sp, g, s, nbr_offsets(g, x, y, z), sw,
                                    n);
   const float rho0 = f ? p.rho0[1] : p.rho0[0];
-  *elem<OFF32>(P, (int64_t)f * g.sp + s) = psi_f(p.psi_form, rho0 + shifted_drho(n));
+  *elem<OFF32>(P + (int64_t)f * g.sp, s) = psi_f(p.psi_form, rho0 + shifted_drho(n));
 }
 
 // Pass 2 for block blk of a lattice; xch: LDS for the partner exchange;
@@ -457,9 +469,9 @@ __device__ inline void ssplit_collide_block(const Store *__restrict__ A, Store *
   using L = D3Q19;
   constexpr int SB = kShiftThreads / 2;
   constexpr bool staged = STAGED;
-  const int w = threadIdx.x >> 6;
+  const int w = uniform(threadIdx.x >> 6);
   const int f = w & 1;
-  const int s = blk * (kShiftThreads / 2) + (w >> 1) * 64 + (threadIdx.x & 63);
+  const int s = uniform(blk * (kShiftThreads / 2) + (w >> 1) * 64) + (threadIdx.x & 63);
   const int k = ((threadIdx.x >> 7) << 6) | (threadIdx.x & 63);
   const uint32_t sw = s >= g.sites ? (1u << 31) : (SOLID ? nbw[s] : 0u);
   const ShiftLane pl = shift_lane(p, f);
@@ -494,10 +506,10 @@ __device__ inline void ssplit_collide_block(const Store *__restrict__ A, Store *
       f, n, rho, rho_p, j, j_p, sw, p, pl,
       [&](int ff, int i) {
         if (staged) return staged_psi<SB>(rows, ff, i, k);
-        return at<OFF32>(P, ff * g.sp + (i < 0 ? s : nbr(i)));
+        return at<OFF32>(P + ff * g.sp, i < 0 ? s : nbr(i));
       },
-      [&](int i) { return (float)at<OFF32>(S, f * g.sp + nbr(i)); },
-      [&](int i, float v) { store_nt<Store>(elem<OFF32>(Bf, (int64_t)i * g.sp + s), v); });
+      [&](int i) { return (float)at<OFF32>(S + f * g.sp, nbr(i)); },
+      [&](int i, float v) { store_nt<Store>(elem<OFF32>(Bf + (int64_t)i * g.sp, s), v); });
 }
 
 template <class Store, bool SOLID, bool OFF32>
