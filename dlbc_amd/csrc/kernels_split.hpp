@@ -154,38 +154,83 @@ __device__ inline void split_pull(const double *__restrict__ Af, const Geo &g, i
   for (int i = 0; i < 19; ++i) n[i] = pull_one<SOLID, OFF32>(Af, g, i, s, o, sw);
 }
 
+// Coordinates of site s of this lane, where the wave's sites are consecutive
+// from s - (lane): the wave's first site is decomposed once with
+// wave-uniform (scalar) arithmetic and each lane adds its offset with at
+// most one carry when n_z >= 64, instead of two 32-bit divisions per lane.
+__device__ inline void lane_coords(int s, const Geo &g, int &x, int &y, int &z) {
+  if (g.nz < 64) {
+    coords(s, g, x, y, z);
+    return;
+  }
+  const int lane = threadIdx.x & 63;
+  int x0, y0, z0;
+  coords(uniform(s - lane), g, x0, y0, z0);
+  x = x0;
+  y = y0;
+  z = z0 + lane;
+  if (z >= g.nz) {
+    z -= g.nz;
+    if (++y == g.ny) {
+      y = 0;
+      ++x;
+    }
+  }
+}
+
 // psi rows of a block of SB consecutive sites that lie in one z-row (n_z a
 // multiple of SB), staged in LDS: the nine rows (x + dx, y + dy), dx, dy in
 // {-1, 0, 1}, from z0 - 1 to z0 + SB (periodic), of both components.  Every
 // neighbour psi (and the solid density in the same array) of the block is in
-// them.  The loads go out first and the LDS stores follow, so that the loads
-// overlap the population pulls issued before; the caller's barrier publishes
-// the rows.
+// them.  Each wave takes whole (component, row) pairs: a wave-uniform row base
+// and consecutive z per lane, no wrap inside the block's span; the 36 end
+// values (z0 - 1 and z0 + SB of each pair) take one pass of wave 0.  The
+// loads go out first and the LDS stores follow, so that the loads overlap
+// the population pulls issued before; the caller's barrier publishes the
+// rows.
 template <int SB, int THREADS, bool OFF32, class T>
 __device__ inline void stage_psi_rows(const T *__restrict__ S, const Geo &g, int s0,
                                       T (*rows)[9][SB + 2]) {
-  constexpr int RW = SB + 2, N = 2 * 9 * RW, NV = (N + THREADS - 1) / THREADS;
+  constexpr int NW = THREADS / 64, NP = 18, PASS = SB / 64, PPW = (NP + NW - 1) / NW;
+  static_assert(SB % 64 == 0, "staged rows take whole lane passes");
   int x0, y0, z0;
-  coords(s0, g, x0, y0, z0);
-  T v[NV];
+  coords(uniform(s0), g, x0, y0, z0);
+  const int w = uniform(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  // element offset of pair pr's row: component pr / 9, row (dx, dy) = pr % 9
+  // (the wrap is arithmetic: a select among three coordinates became a
+  // lookup table in scratch memory)
+  auto pair_base = [&](int pr) {
+    const int row = pr % 9;
+    const int xx = wrap(x0 + row / 3 - 1, g.nx), yy = wrap(y0 + row % 3 - 1, g.ny);
+    return (int64_t)(pr / 9) * g.sp + (int64_t)(xx * g.ny + yy) * g.nz;
+  };
+  T v[PPW * PASS];
 #pragma unroll
-  for (int r = 0; r < NV; ++r) {
-    const int e = threadIdx.x + r * THREADS;
-    if (e < N) {
-      const int comp = e / (9 * RW);
-      const int rem = e - comp * 9 * RW;
-      const int row = rem / RW;
-      const int zo = rem - row * RW;
-      const int dx = row / 3 - 1, dy = row - (row / 3) * 3 - 1;
-      const int nb = site_of(wrap(x0 + dx, g.nx), wrap(y0 + dy, g.ny), wrap(z0 - 1 + zo, g.nz), g);
-      v[r] = at<OFF32>(S, (int64_t)comp * g.sp + nb);
+  for (int q = 0; q < PPW; ++q) {
+    const int pr = w + q * NW;
+    const T *base = S + pair_base(pr < NP ? pr : 0) + z0;
+#pragma unroll
+    for (int k = 0; k < PASS; ++k)
+      v[q * PASS + k] = pr < NP ? at<OFF32>(base, lane + 64 * k) : T(0);
+  }
+  // the two ends of every pair's span: lane 2 pr + side of wave 0
+  const bool edge = w == 0 && lane < 2 * NP;
+  const int epr = lane >> 1, side = lane & 1;
+  T ve = T(0);
+  if (edge) {
+    const int ze = side ? (z0 + SB == g.nz ? 0 : z0 + SB) : (z0 == 0 ? g.nz - 1 : z0 - 1);
+    ve = at<OFF32>(S, pair_base(epr) + ze);
+  }
+#pragma unroll
+  for (int q = 0; q < PPW; ++q) {
+    const int pr = w + q * NW;
+    if (pr < NP) {
+      T *row = rows[pr / 9][pr % 9] + 1 + lane;
+#pragma unroll
+      for (int k = 0; k < PASS; ++k) row[64 * k] = v[q * PASS + k];
     }
   }
-#pragma unroll
-  for (int r = 0; r < NV; ++r) {
-    const int e = threadIdx.x + r * THREADS;
-    if (e < N) (&rows[0][0][0])[e] = v[r];
-  }
+  if (edge) rows[epr / 9][epr % 9][side ? SB + 1 : 0] = ve;
 }
 
 // The staged value of component ff at x + c_i for site k of the block (i < 0: x).
@@ -205,7 +250,7 @@ __global__ void __launch_bounds__(kSplitThreads)
   const uint32_t sw = SOLID ? nbw[s] : 0u;
   if (sw >> 31) return;  // solid: S keeps the site's density
   int x, y, z;
-  coords(s, g, x, y, z);
+  lane_coords(s, g, x, y, z);
   double n[19];
   split_pull<SOLID, OFF32>(A + (int64_t)f * 19 * g.sp, g, s, nbr_offsets(g, x, y, z), sw, n);
   *elem<OFF32>(S + (int64_t)f * g.sp, s) = psi_of(psi_form, density<D3Q19>(n));
@@ -371,7 +416,7 @@ __global__ void __launch_bounds__(kSplitThreads)
   const bool bdist2 = p.eq_form == kBDist2;
   if (!(sw >> 31)) {
     int x, y, z;
-    coords(s, g, x, y, z);
+    lane_coords(s, g, x, y, z);
     o = nbr_offsets(g, x, y, z);
     split_pull<SOLID, OFF32>(A + (int64_t)f * 19 * g.sp, g, s, o, sw, n);
   }
@@ -445,7 +490,7 @@ __device__ inline void ssplit_psi_block(const Store *__restrict__ A,
   const uint32_t sw = SOLID ? nbw[s] : 0u;
   if (sw >> 31) return;  // solid: psi is not read there
   int x, y, z;
-  coords(s, g, x, y, z);
+  lane_coords(s, g, x, y, z);
   float n[19];
   ssplit_pull<Store, SOLID, OFF32>(A + (int64_t)f * 19 * g.sp, g, s, nbr_offsets(g, x, y, z), sw,
                                    n);
@@ -480,7 +525,7 @@ __device__ inline void ssplit_collide_block(const Store *__restrict__ A, Store *
   float rho = 0.0f, j[3] = {0.0f, 0.0f, 0.0f};
   if (!(sw >> 31)) {
     int x, y, z;
-    coords(s, g, x, y, z);
+    lane_coords(s, g, x, y, z);
     o = nbr_offsets(g, x, y, z);
     ssplit_pull<Store, SOLID, OFF32>(A + (int64_t)f * 19 * g.sp, g, s, o, sw, n);
   }
