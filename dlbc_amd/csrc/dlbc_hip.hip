@@ -111,8 +111,7 @@ inline int blocks_for(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
 // geometry, 0 keeps site order, K > 0 gives the XCDs K blocks per turn.  The
 // automatic choice is site order (see kSplitAutoChunk).
 void apply_block_order(dlbc_gpu_ctx *c, int chunk) {
-  const int threads = c->storage == DLBC_STORAGE_F64 ? kSplitThreads : kShiftThreads;
-  const int spb = threads / 2;
+  const int spb = c->storage == DLBC_STORAGE_F64 ? kSplitThreads / 2 : kShiftSites;
   if (chunk < 0) {
     chunk = 0;
     const int64_t plane = (int64_t)c->n[1] * c->n[2];
@@ -120,7 +119,7 @@ void apply_block_order(dlbc_gpu_ctx *c, int chunk) {
       chunk = (int)(plane / spb / 8);  // an eighth of an x-plane per turn
   }
   c->geo.order_chunk = chunk;
-  c->geo.order_main = split_order_main(split_grid(c->geo.sites, threads), chunk);
+  c->geo.order_main = split_order_main((int)((c->geo.sites + spb - 1) / spb), chunk);
 }
 
 // ---- population loaders -----------------------------------------------------
@@ -140,7 +139,7 @@ struct LoadShift {
   int64_t sp;
   double rho0[4];
   __device__ double operator()(int f, int i, int s) const {
-    return D3Q19::w[i] * rho0[f] + (double)to_f(A[(int64_t)(f * 19 + i) * sp + s]);
+    return D3Q19::w[i] * rho0[f] + (double)to_f(A[shift_elem(f, i, s, sp)]);
   }
 };
 
@@ -155,8 +154,9 @@ __global__ void k_aos_to_soa(const double *__restrict__ stg, St *__restrict__ A,
   if (t >= cnt) return;
   for (int i = 0; i < q; ++i) {
     const St sv = to_storage<St>(stg[(int64_t)t * q + i], i, rho0);
-    A[(int64_t)(f * q + i) * sp + s0 + t] = sv;
-    B[(int64_t)(f * q + i) * sp + s0 + t] = sv;
+    const int64_t o = pop_elem<St>(f, i, s0 + t, q, sp);
+    A[o] = sv;
+    B[o] = sv;
   }
 }
 
@@ -195,10 +195,14 @@ __global__ void k_fill_default(St *__restrict__ A, St *__restrict__ B, int64_t n
 // at every solid site.
 template <class St>
 __global__ void k_copy_solid(const St *__restrict__ A, St *__restrict__ B,
-                             const uint8_t *__restrict__ mask, Geo g, int nq) {
+                             const uint8_t *__restrict__ mask, Geo g, int C, int q) {
   const int s = blockIdx.x * 256 + threadIdx.x;
   if (s >= g.sites || !mask[s]) return;
-  for (int k = 0; k < nq; ++k) B[(int64_t)k * g.sp + s] = A[(int64_t)k * g.sp + s];
+  for (int f = 0; f < C; ++f)
+    for (int i = 0; i < q; ++i) {
+      const int64_t o = pop_elem<St>(f, i, s, q, g.sp);
+      B[o] = A[o];
+    }
 }
 
 // Solid-neighbour word of every site: bit i = solid(x + c_i), bit 31 =
@@ -1066,19 +1070,18 @@ static int mask_ready(dlbc_gpu_ctx *c, int solid) {
   }
   if (solid) {
     const int nb = blocks_for(c->geo.sites, 256);
-    const int nq = c->C * c->q;
     if (c->storage == DLBC_STORAGE_F64)
       hipLaunchKernelGGL(k_copy_solid<double>, dim3(nb), dim3(256), 0, c->stream,
                          static_cast<const double *>(c->popA), static_cast<double *>(c->popB),
-                         c->mask, c->geo, nq);
+                         c->mask, c->geo, c->C, c->q);
     else if (c->storage == DLBC_STORAGE_F32_SHIFT)
       hipLaunchKernelGGL(k_copy_solid<float>, dim3(nb), dim3(256), 0, c->stream,
                          static_cast<const float *>(c->popA), static_cast<float *>(c->popB),
-                         c->mask, c->geo, nq);
+                         c->mask, c->geo, c->C, c->q);
     else
       hipLaunchKernelGGL(k_copy_solid<__half>, dim3(nb), dim3(256), 0, c->stream,
                          static_cast<const __half *>(c->popA), static_cast<__half *>(c->popB),
-                         c->mask, c->geo, nq);
+                         c->mask, c->geo, c->C, c->q);
     HIPCHK(c, hipGetLastError());
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1858,8 +1861,7 @@ int dlbc_batch_create(dlbc_batch **out, dlbc_gpu_ctx *const *lattices, int n) {
   if (hipMalloc(&b->d_lats, sizeof(BatchLattice) * n) != hipSuccess) return bail(DLBC_ERR_NOMEM);
   b->h_first.assign(1, 0);
   for (int l = 0; l < n; ++l)
-    b->h_first.push_back(b->h_first.back() + (lattices[l]->geo.sites + kShiftThreads / 2 - 1) /
-                                                 (kShiftThreads / 2));
+    b->h_first.push_back(b->h_first.back() + shift_grid(lattices[l]->geo.sites));
   if (hipMalloc(&b->d_first, sizeof(int) * (n + 1)) != hipSuccess) return bail(DLBC_ERR_NOMEM);
   if (hipMemcpy(b->d_first, b->h_first.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice) !=
       hipSuccess)

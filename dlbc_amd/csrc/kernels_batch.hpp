@@ -115,8 +115,9 @@ __device__ __forceinline__ void batch_tile(const BatchItem &item, ConstLattice &
     const int ring = FULL ? Sh::ring : 2 * (vz + 2) + 2 * vy;
     const int ring_items = ring * 2 * Q;
     const int y = y0 + ly, z = z0 + lz;
-    const Store *__restrict__ Af = A + (int64_t)f * Q * sp;
-    Store *__restrict__ Wf = W + (int64_t)f * Q * sp;
+    // component f's populations: elements shift_elem(f, i, s) = 2 (i sp + s) + f
+    const Store *__restrict__ Af = A + f;
+    Store *__restrict__ Wf = W + f;
     const float rho0 = pl.rho0;
 
     auto gsite = [&](int x, int yy, int zz) { return (x * ny + yy) * nz + zz; };
@@ -139,7 +140,7 @@ __device__ __forceinline__ void batch_tile(const BatchItem &item, ConstLattice &
         const bool bb = (sw >> L::bounce[i]) & 1u;
         const int comp = bb ? L::bounce[i] : i;
         const int src = bb ? self : gsite(sx, sy, sz);
-        n[i] = to_f(Af[(int64_t)comp * sp + src]);
+        n[i] = to_f(Af[2 * ((int64_t)comp * sp + src)]);
       }
     };
     auto drho = [](const float *n) { return shifted_drho(n); };
@@ -163,7 +164,7 @@ __device__ __forceinline__ void batch_tile(const BatchItem &item, ConstLattice &
           const int comp = bb ? bi : i;
           const int src =
               bb ? self : gsite(wrapf(x - cx, nx), wrapf(yy - cy, ny), wrapf(zz - cz, nz));
-          v = to_f(A[(int64_t)(fr * Q + comp) * sp + src]);
+          v = to_f(A[shift_elem(fr, comp, src, sp)]);
         }
         stg[w] = v;
       }
@@ -246,7 +247,7 @@ __device__ __forceinline__ void batch_tile(const BatchItem &item, ConstLattice &
                                      wrapf(z + L::c[i][2], nz));
                 return (float)B.S[f * sp + nb];
               },
-              [&](int i, float v) { store_nt<Store>(&Wf[(int64_t)i * sp + s], v); });
+              [&](int i, float v) { store_nt<Store>(&Wf[2 * ((int64_t)i * sp + s)], v); });
         }
       }
 #pragma unroll
@@ -372,7 +373,6 @@ template <class Store>
 __global__ void __launch_bounds__(kShiftThreads)
     k_batch_split_collide(const BatchLattice *__restrict__ lats, const int *__restrict__ first,
                           int nlat, int blk0, int blk1, int parity) {
-  __shared__ float xch[2][4][kShiftThreads / 2];
   const int b = blk0 + blockIdx.x;
   if (b >= blk1) return;  // whole block: no barrier is left waiting
   const int l = batch_block_lattice(first, nlat, b);
@@ -382,10 +382,9 @@ __global__ void __launch_bounds__(kShiftThreads)
   const Geo g = B.g;
   const ShiftParams p = B.p;
   if (p.has_solid)
-    ssplit_collide_block<Store, true, false>(A, W, B.nbw, B.S, B.P, g, p, b - first[l], xch,
-                                             nullptr);
+    ssplit_collide_block<Store, true, false>(A, W, B.nbw, B.S, B.P, g, p, b - first[l], nullptr);
   else
-    ssplit_collide_block<Store, false, false>(A, W, B.nbw, B.S, B.P, g, p, b - first[l], xch,
+    ssplit_collide_block<Store, false, false>(A, W, B.nbw, B.S, B.P, g, p, b - first[l],
                                               nullptr);
 }
 

@@ -30,6 +30,17 @@ __device__ inline St to_storage(double v, int i, double rho0) {
   }
 }
 
+// Element of population i of component f at site s: FP64 storage is
+// [C][q][sp]; shifted storage keeps the two components side by side
+// (shift_elem).
+template <class St>
+__device__ inline int64_t pop_elem(int f, int i, int s, int q, int64_t sp) {
+  if constexpr (std::is_same<St, double>::value)
+    return (int64_t)(f * q + i) * sp + s;
+  else
+    return shift_elem(f, i, s, sp);
+}
+
 // Where a site's value comes from.
 enum InitMode : int {
   kInitConst = 0,  // value * u_i (Const with u = 1, EqDist)
@@ -64,7 +75,7 @@ __global__ void __launch_bounds__(256)
   }
   for (int i = 0; i < a.q; ++i) {
     const double v = a.mode == kInitPop ? vals[(int64_t)t * a.q + i] : val * a.u[i];
-    const int64_t o = (int64_t)(a.f * a.q + i) * g.sp + s;
+    const int64_t o = pop_elem<St>(a.f, i, s, a.q, g.sp);
     const St sv = to_storage<St>(v, i, a.rho0);
     A[o] = sv;
     B[o] = sv;
@@ -79,7 +90,7 @@ __global__ void __launch_bounds__(256)
   const int s = blockIdx.x * 256 + threadIdx.x;
   if (s >= g.sites || !mask[s]) return;
   for (int i = 0; i < q; ++i) {
-    const int64_t o = (int64_t)(f * q + i) * g.sp + s;
+    const int64_t o = pop_elem<St>(f, i, s, q, g.sp);
     const St sv = to_storage<St>(i == 0 ? 1.0 : 0.0, i, rho0);
     A[o] = sv;
     B[o] = sv;

@@ -79,6 +79,44 @@ __device__ inline void store_nt<__half>(__half *p, float v) {
                               reinterpret_cast<unsigned short *>(p));
 }
 
+// Layout of a shifted-storage buffer: the two components of a population sit
+// side by side, element (f, i, s) at 2 (i sp + s) + f, so one 8-byte (FP32)
+// or 4-byte (FP16) access moves both components of a site.
+__host__ __device__ inline int64_t shift_elem(int f, int i, int64_t s, int64_t sp) {
+  return 2 * ((int64_t)i * sp + s) + f;
+}
+
+// Both components at p (an even element) as FP32.
+template <class T>
+__device__ inline void pair_load(const T *p, float &a, float &b);
+template <>
+__device__ inline void pair_load<float>(const float *p, float &a, float &b) {
+  const float2 v = *reinterpret_cast<const float2 *>(p);
+  a = v.x;
+  b = v.y;
+}
+template <>
+__device__ inline void pair_load<__half>(const __half *p, float &a, float &b) {
+  const __half2 v = *reinterpret_cast<const __half2 *>(p);
+  a = __low2float(v);
+  b = __high2float(v);
+}
+
+// Streaming (nontemporal) store of both components at p (an even element).
+template <class T>
+__device__ inline void pair_store_nt(T *p, float a, float b);
+template <>
+__device__ inline void pair_store_nt<float>(float *p, float a, float b) {
+  const uint64_t v = ((uint64_t)__float_as_uint(b) << 32) | __float_as_uint(a);
+  __builtin_nontemporal_store(v, reinterpret_cast<uint64_t *>(p));
+}
+template <>
+__device__ inline void pair_store_nt<__half>(__half *p, float a, float b) {
+  const uint32_t v = ((uint32_t)__half_as_ushort(__float2half_rn(materialise(b))) << 16) |
+                     __half_as_ushort(__float2half_rn(materialise(a)));
+  __builtin_nontemporal_store(v, reinterpret_cast<uint32_t *>(p));
+}
+
 struct ShiftParams {
   float tau[2], omega[2], rho0[2];
   float gcc[2][2], gwc[2];

@@ -34,9 +34,10 @@
 namespace dlbc {
 
 constexpr int kSplitThreads = 256;  // FP64: 4 waves = 128 sites x 2 components
-// Shifted storage: 8 waves = 256 sites x 2 components (measured: FP32 1.85 ms
-// per 256^3 step against 1.92 ms with 4 waves; FP64 prefers 4: 3.41 vs 3.50).
-constexpr int kShiftThreads = 512;
+// Shifted storage: one lane per site holding both components (their
+// populations sit side by side), 4 waves = 256 sites.
+constexpr int kShiftThreads = 256;
+constexpr int kShiftSites = 256;
 
 // Grid of a split launch over `sites`: threads / 2 sites per block.  (A grid
 // that gave each XCD one contiguous run of sites, an eighth of the lattice,
@@ -470,91 +471,106 @@ __global__ void __launch_bounds__(kSplitThreads)
 // in the tiled kernel, so a lattice gives the same bits under both schedules.
 // S still gives the density at solid neighbours (wall force).
 
+// Both components' streamed populations at site s (pull with bounce-back,
+// src/dlbc/lb/advection.d:61-89): one 8-byte (FP32) or 4-byte (FP16) load
+// per population and lane, from a wave-uniform base, array min(i, bounce(i)),
+// at a lane offset below 16 sp elements.
 template <class Store, bool SOLID, bool OFF32>
-__device__ inline void ssplit_pull(const Store *__restrict__ Af, const Geo &g, int s,
-                                   const NbrOffsets &o, uint32_t sw, float *n) {
+__device__ inline void ssplit_pull(const Store *__restrict__ A, const Geo &g, int s,
+                                   const NbrOffsets &o, uint32_t sw, float (&n0)[19],
+                                   float (&n1)[19]) {
+  using L = D3Q19;
 #pragma unroll
-  for (int i = 0; i < 19; ++i) n[i] = to_f(pull_one<SOLID, OFF32>(Af, g, i, s, o, sw));
+  for (int i = 0; i < 19; ++i) {
+    const int src = nbr_site(s, o, -L::c[i][0], -L::c[i][1], -L::c[i][2]);
+    const Store *p;
+    if (!SOLID) {
+      p = elem<OFF32>(A + 2 * (int64_t)i * g.sp, 2 * (int64_t)src);
+    } else {
+      const int b = L::bounce[i], mc = i < b ? i : b;
+      const bool bb = (sw >> b) & 1u;
+      p = elem<OFF32>(A + 2 * (int64_t)mc * g.sp,
+                      2 * ((int64_t)((bb ? b : i) - mc) * g.sp + (bb ? s : src)));
+    }
+    pair_load(p, n0[i], n1[i]);
+  }
 }
 
-// Pass 1 for block blk of a lattice (the sites blk * 128 ..).
+// The sites of block blk of a lattice (kShiftSites from blk * kShiftSites),
+// one per lane.
+__device__ inline int shift_site(int blk) {
+  return uniform(blk * kShiftSites + (threadIdx.x & ~63)) + (threadIdx.x & 63);
+}
+
+// Pass 1 for block blk of a lattice: psi of both components into P.
 template <class Store, bool SOLID, bool OFF32>
 __device__ inline void ssplit_psi_block(const Store *__restrict__ A,
                                         const uint32_t *__restrict__ nbw,
                                         float *__restrict__ P, const Geo &g,
                                         const ShiftParams &p, int blk) {
-  const int w = uniform(threadIdx.x >> 6);
-  const int f = w & 1;
-  const int s = uniform(blk * (kShiftThreads / 2) + (w >> 1) * 64) + (threadIdx.x & 63);
+  const int s = shift_site(blk);
   if (s >= g.sites) return;
   const uint32_t sw = SOLID ? nbw[s] : 0u;
   if (sw >> 31) return;  // solid: psi is not read there
   int x, y, z;
   lane_coords(s, g, x, y, z);
-  float n[19];
-  ssplit_pull<Store, SOLID, OFF32>(A + (int64_t)f * 19 * g.sp, g, s, nbr_offsets(g, x, y, z), sw,
-                                   n);
-  const float rho0 = f ? p.rho0[1] : p.rho0[0];
-  *elem<OFF32>(P + (int64_t)f * g.sp, s) = psi_f(p.psi_form, rho0 + shifted_drho(n));
+  float n0[19], n1[19];
+  ssplit_pull<Store, SOLID, OFF32>(A, g, s, nbr_offsets(g, x, y, z), sw, n0, n1);
+  *elem<OFF32>(P, s) = psi_f(p.psi_form, p.rho0[0] + shifted_drho(n0));
+  *elem<OFF32>(P + g.sp, s) = psi_f(p.psi_form, p.rho0[1] + shifted_drho(n1));
 }
 
-// Pass 2 for block blk of a lattice; xch: LDS for the partner exchange;
-// rows: LDS for the block's psi rows when STAGED (n_z a multiple of the
-// block's sites).  STAGED is a template argument, not a runtime test: with
-// both psi sources in one body the compiler merged the LDS and global reads
-// into flat loads, each waiting for all memory traffic of the wave.
+// Pass 2 for block blk of a lattice; rows: LDS for the block's psi rows when
+// STAGED (n_z a multiple of kShiftSites).  STAGED is a template argument, not
+// a runtime test: with both psi sources in one body the compiler merged the
+// LDS and global reads into flat loads, each waiting for all memory traffic
+// of the wave.  A lane collides component 0 (keeping the result in place of
+// its populations) and then component 1, and stores each population of both
+// components with one access.
 template <class Store, bool SOLID, bool OFF32, bool STAGED = false>
 __device__ inline void ssplit_collide_block(const Store *__restrict__ A, Store *__restrict__ B,
                                             const uint32_t *__restrict__ nbw,
                                             const double *__restrict__ S,
                                             const float *__restrict__ P, const Geo &g,
                                             const ShiftParams &p, int blk,
-                                            float (*xch)[4][kShiftThreads / 2],
-                                            float (*rows)[9][kShiftThreads / 2 + 2]) {
+                                            float (*rows)[9][kShiftSites + 2]) {
   using L = D3Q19;
-  constexpr int SB = kShiftThreads / 2;
+  constexpr int SB = kShiftSites;
   constexpr bool staged = STAGED;
-  const int w = uniform(threadIdx.x >> 6);
-  const int f = w & 1;
-  const int s = uniform(blk * (kShiftThreads / 2) + (w >> 1) * 64) + (threadIdx.x & 63);
-  const int k = ((threadIdx.x >> 7) << 6) | (threadIdx.x & 63);
+  const int s = shift_site(blk);
+  const int k = threadIdx.x;  // site within the block
   const uint32_t sw = s >= g.sites ? (1u << 31) : (SOLID ? nbw[s] : 0u);
-  const ShiftLane pl = shift_lane(p, f);
   NbrOffsets o{};
-  float n[19];
-  float rho = 0.0f, j[3] = {0.0f, 0.0f, 0.0f};
+  float n0[19], n1[19];
   if (!(sw >> 31)) {
     int x, y, z;
     lane_coords(s, g, x, y, z);
     o = nbr_offsets(g, x, y, z);
-    ssplit_pull<Store, SOLID, OFF32>(A + (int64_t)f * 19 * g.sp, g, s, o, sw, n);
+    ssplit_pull<Store, SOLID, OFF32>(A, g, s, o, sw, n0, n1);
   }
-  if (staged) stage_psi_rows<SB, kShiftThreads, OFF32>(P, g, blk * SB, rows);
-  if (!(sw >> 31)) {
-    rho = pl.rho0 + shifted_drho(n);
-    shifted_moment(n, j);
+  if (staged) {
+    stage_psi_rows<SB, kShiftThreads, OFF32>(P, g, blk * SB, rows);
+    __syncthreads();
   }
-  xch[f][0][k] = rho;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) xch[f][1 + d][k] = j[d];
-  __syncthreads();
   if (sw >> 31) return;  // solid populations stay as they are
-  const float rho_p = xch[f ^ 1][0][k];
-  float j_p[3] = {0.0f, 0.0f, 0.0f};
-  if (p.eq_form == kBDist2) {
-#pragma unroll
-    for (int d = 0; d < 3; ++d) j_p[d] = xch[f ^ 1][1 + d][k];
-  }
+  const ShiftLane pl0 = shift_lane(p, 0), pl1 = shift_lane(p, 1);
+  const float rho0 = pl0.rho0 + shifted_drho(n0), rho1 = pl1.rho0 + shifted_drho(n1);
+  float j0[3], j1[3];
+  shifted_moment(n0, j0);
+  shifted_moment(n1, j1);
   auto nbr = [&](int i) { return nbr_site(s, o, L::c[i][0], L::c[i][1], L::c[i][2]); };
-  Store *__restrict__ Bf = B + (int64_t)f * 19 * g.sp;
+  auto psi = [&](int ff, int i) {
+    if (staged) return staged_psi<SB>(rows, ff, i, k);
+    return at<OFF32>(P + ff * g.sp, i < 0 ? s : nbr(i));
+  };
   shifted_collide_site<SOLID>(
-      f, n, rho, rho_p, j, j_p, sw, p, pl,
-      [&](int ff, int i) {
-        if (staged) return staged_psi<SB>(rows, ff, i, k);
-        return at<OFF32>(P + ff * g.sp, i < 0 ? s : nbr(i));
-      },
-      [&](int i) { return (float)at<OFF32>(S + f * g.sp, nbr(i)); },
-      [&](int i, float v) { store_nt<Store>(elem<OFF32>(Bf + (int64_t)i * g.sp, s), v); });
+      0, n0, rho0, rho1, j0, j1, sw, p, pl0, psi,
+      [&](int i) { return (float)at<OFF32>(S, nbr(i)); }, [&](int i, float v) { n0[i] = v; });
+  shifted_collide_site<SOLID>(
+      1, n1, rho1, rho0, j1, j0, sw, p, pl1, psi,
+      [&](int i) { return (float)at<OFF32>(S + g.sp, nbr(i)); }, [&](int i, float v) {
+        pair_store_nt<Store>(elem<OFF32>(B + 2 * (int64_t)i * g.sp, 2 * (int64_t)s), n0[i], v);
+      });
 }
 
 template <class Store, bool SOLID, bool OFF32>
@@ -569,17 +585,18 @@ __global__ void __launch_bounds__(kShiftThreads)
     k_ssplit_collide(const Store *__restrict__ A, Store *__restrict__ B,
                      const uint32_t *__restrict__ nbw, const double *__restrict__ S,
                      const float *__restrict__ P, Geo g, ShiftParams p) {
-  __shared__ float xch[2][4][kShiftThreads / 2];
-  __shared__ float rows[STAGED ? 2 : 1][9][kShiftThreads / 2 + 2];
-  ssplit_collide_block<Store, SOLID, OFF32, STAGED>(A, B, nbw, S, P, g, p, split_block(g), xch,
-                                                    rows);
+  __shared__ float rows[STAGED ? 2 : 1][9][kShiftSites + 2];
+  ssplit_collide_block<Store, SOLID, OFF32, STAGED>(A, B, nbw, S, P, g, p, split_block(g), rows);
 }
+
+// Blocks of kShiftSites sites over a lattice.
+inline int shift_grid(int64_t sites) { return (int)((sites + kShiftSites - 1) / kShiftSites); }
 
 template <class Store, bool OFF32>
 inline void launch_ssplit_impl(const Store *a, Store *b, const uint32_t *nbw, const double *S,
                                float *P, const Geo &g, const ShiftParams &p,
                                hipStream_t stream, int pass) {
-  const int nb = split_grid(g.sites, kShiftThreads);
+  const int nb = shift_grid(g.sites);
   const dim3 grid(nb), block(kShiftThreads);
   if (pass == 0) {
     if (p.has_solid)
@@ -589,7 +606,7 @@ inline void launch_ssplit_impl(const Store *a, Store *b, const uint32_t *nbw, co
       hipLaunchKernelGGL((k_ssplit_psi<Store, false, OFF32>), grid, block, 0, stream, a, nbw, P,
                          g, p);
   } else {
-    const bool staged = g.nz % (kShiftThreads / 2) == 0;
+    const bool staged = g.nz % kShiftSites == 0;
     if (p.has_solid && staged)
       hipLaunchKernelGGL((k_ssplit_collide<Store, true, OFF32, true>), grid, block, 0, stream, a,
                          b, nbw, S, P, g, p);

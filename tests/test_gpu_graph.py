@@ -213,3 +213,30 @@ def test_mask_change_mid_run_matches_oracle(D, oracle_mod):
         for f in range(2):
             assert np.array_equal(L.population(f).view(np.uint64),
                                   o.population(f).view(np.uint64)), (what, f)
+
+
+@pytest.mark.parametrize("storage", ["f32_shift", "f16_shift"])
+def test_mask_change_mid_run_shifted_keeps_solid_state(D, storage):
+    """Shifted storage: walls added after 17 steps (the two buffers then hold
+    different states) keep, at every new solid site, the populations the site
+    had when the walls came, in both buffers (an even and an odd number of
+    steps later), and graph replays equal eager steps bit for bit."""
+    p = _walls((16, 20, 24))
+    a = D.Lattice.from_params(p, storage=storage)
+    b = D.Lattice.from_params(p, storage=storage)
+    a.step(17)
+    b.step(17, graph=True)
+    before = [a.population(f) for f in range(2)]
+    walls = [[False, False], [True, True], [False, False]]
+    old = a.mask()
+    for L in (a, b):
+        L.init_mask("Walls", dim_has_wall=walls)
+    new = (a.mask() != 0) & (old == 0)
+    assert new.any()
+    for steps in (40, 1):
+        a.step(steps)
+        b.step(steps, graph=True)
+        for f in range(2):
+            got = a.population(f)[new]
+            assert np.array_equal(got.view(np.uint64), before[f][new].view(np.uint64)), (steps, f)
+        _same(a, b, "shifted after init_mask, %d steps" % steps)
