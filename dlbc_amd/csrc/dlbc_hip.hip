@@ -225,14 +225,16 @@ __global__ void k_solid_words(const uint8_t *__restrict__ mask, uint32_t *__rest
 
 template <class L, class Ld>
 __global__ void k_solid_rho(Ld ld, const uint8_t *__restrict__ mask, double *__restrict__ S,
-                            int C, Geo g) {
+                            float *__restrict__ P, int C, Geo g) {
   const int s = blockIdx.x * 256 + threadIdx.x;
   if (s >= g.sites || !mask[s]) return;
   for (int f = 0; f < C; ++f) {
     double n[L::q];
 #pragma unroll
     for (int i = 0; i < L::q; ++i) n[i] = ld(f, i, s);
-    S[f * g.sp + s] = density<L>(n);
+    const double rho = density<L>(n);
+    S[f * g.sp + s] = rho;
+    if (P) P[f * g.sp + s] = (float)rho;
   }
 }
 
@@ -546,10 +548,10 @@ struct SplitStep {
                              static_cast<double *>(c->popB), c->nbw, c->S, c->geo, c->p,
                              c->stream, k);
         else if (c->storage == DLBC_STORAGE_F32_SHIFT)
-          launch_ssplit_d3q19<float>(c->popA, c->popB, c->nbw, c->S, c->P, c->geo, c->sparams,
+          launch_ssplit_d3q19<float>(c->popA, c->popB, c->nbw, c->P, c->geo, c->sparams,
                                      c->stream, k);
         else
-          launch_ssplit_d3q19<__half>(c->popA, c->popB, c->nbw, c->S, c->P, c->geo, c->sparams,
+          launch_ssplit_d3q19<__half>(c->popA, c->popB, c->nbw, c->P, c->geo, c->sparams,
                                       c->stream, k);
       };
       if (c->p.sc) {
@@ -621,7 +623,7 @@ int prepare(dlbc_gpu_ctx *c) {
       const int nb = blocks_for(c->geo.sites, 256);
       with_loader(c, [&](auto lat, auto ld) {
         hipLaunchKernelGGL((k_solid_rho<decltype(lat), decltype(ld)>), dim3(nb), dim3(256), 0,
-                           c->stream, ld, c->mask, c->S, c->C, c->geo);
+                           c->stream, ld, c->mask, c->S, c->P, c->C, c->geo);
         return 0;
       });
       HIPCHK(c, hipGetLastError());

@@ -242,10 +242,10 @@ __device__ __forceinline__ void batch_tile(const BatchItem &item, ConstLattice &
                 if (i < 0) return psi_lds[pslot(x)][ff][me_e];
                 return psi_lds[pslot(x + L::c[i][0])][ff][me_e + L::c[i][1] * EZ + L::c[i][2]];
               },
-              [&](int i) {
+              [&](int i, bool b) {
                 const int nb = gsite(wrapf(x + L::c[i][0], nx), wrapf(y + L::c[i][1], ny),
                                      wrapf(z + L::c[i][2], nz));
-                return (float)B.S[f * sp + nb];
+                return B.P[f * sp + (b ? nb : s)];  // the density at solid sites
               },
               [&](int i, float v) { store_nt<Store>(&Wf[2 * ((int64_t)i * sp + s)], v); });
         }
@@ -382,9 +382,9 @@ __global__ void __launch_bounds__(kShiftThreads)
   const Geo g = B.g;
   const ShiftParams p = B.p;
   if (p.has_solid)
-    ssplit_collide_block<Store, true, false>(A, W, B.nbw, B.S, B.P, g, p, b - first[l], nullptr);
+    ssplit_collide_block<Store, true, false>(A, W, B.nbw, B.P, g, p, b - first[l], nullptr);
   else
-    ssplit_collide_block<Store, false, false>(A, W, B.nbw, B.S, B.P, g, p, b - first[l],
+    ssplit_collide_block<Store, false, false>(A, W, B.nbw, B.P, g, p, b - first[l],
                                               nullptr);
 }
 

@@ -137,8 +137,14 @@ __device__ inline float ffmul(float x, FF c) { return fmaf(x, c.hi, x * c.lo); }
 // D3Q19 weight of population i (connectivity.d:252-256) as a float-float pair.
 __device__ inline FF d3q19_wff(int i) { return i == 0 ? kW0 : (i < 7 ? kW1 : kW2); }
 
-// Deviation of the unit-density equilibrium, E_i(u) - w_i, for D3Q19.
-__device__ inline void eq_dev_d3q19(int form, const float (&u)[3], float (&e)[19]) {
+// Deviation of the equilibrium from the reference state,
+//   d_i^eq = w_i (rho - rho0) + rho (E_i(u) - w_i),
+// for D3Q19, with E_i(u) the equilibrium of unit density; drh = rho - rho0.
+// The weight classes share their products: w_i drh takes one float-float
+// product per class (w_0 = 1/3, w_1..6 = 1/18, w_7..18 = 1/36), and BDist2's
+// 216 (E_i - w_i) (exact constants) meets rho / 216, formed once, in one FMA.
+__device__ inline void eq_dev_d3q19(int form, const float (&u)[3], float rho, float drh,
+                                    float (&e)[19]) {
   using L = D3Q19;
   if (form != kBDist2) {
     const float uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
@@ -147,13 +153,13 @@ __device__ inline void eq_dev_d3q19(int form, const float (&u)[3], float (&e)[19
       const float cu = (float)L::c[i][0] * u[0] + (float)L::c[i][1] * u[1] + (float)L::c[i][2] * u[2];
       float t = fmaf(4.5f * cu, cu, fmaf(3.0f, cu, -1.5f * uu));
       if (form == kThirdOrder) t += (4.5f * cu) * fmaf(cu, cu, -uu);
-      e[i] = ffmul(t, d3q19_wff(i));
+      e[i] = ffmul(fmaf(rho, t, drh), d3q19_wff(i));
     }
     return;
   }
   // BDist2 (eqdist.d:102-184) with the constant part w_i removed, scaled by
   // 216 so that every constant is exact (aN1 = 1/36 -> 6, aN1 / 2 -> 3,
-  // aN1 / 6 -> 1, 1/3 -> 72), then multiplied by 1/216 as a float-float pair.
+  // aN1 / 6 -> 1, 1/3 -> 72)
   const float tx = 3.0f * u[0], ty = 3.0f * u[1], tz = 3.0f * u[2];
   const float uke = 0.5f * (u[0] * tx + u[1] * ty + u[2] * tz);
   const float c1 = 6.0f * (1.0f - uke);
@@ -191,8 +197,45 @@ __device__ inline void eq_dev_d3q19(int form, const float (&u)[3], float (&e)[19
   E[15] = ev + od;
   E[10] = ev - od;
   E[0] = -72.0f * uke;
+  const float r216 = ffmul(rho, kInv216);
+  const float dw0 = ffmul(drh, kW0), dw1 = ffmul(drh, kW1), dw2 = ffmul(drh, kW2);
 #pragma unroll
-  for (int i = 0; i < 19; ++i) e[i] = ffmul(E[i], kInv216);
+  for (int i = 0; i < 19; ++i) e[i] = fmaf(r216, E[i], i == 0 ? dw0 : (i < 7 ? dw1 : dw2));
+}
+
+// All ones when bit i of w is set, else zero.  The empty asm hides that the
+// mask is one of two values: the compiler would otherwise turn the blends
+// below back into selects and then move the load of the unused operand into a
+// branch of its own (one branch and one wait per neighbour).
+__device__ inline uint32_t bit_mask(uint32_t w, int i) {
+  uint32_t m = 0u - ((w >> i) & 1u);
+  asm("" : "+v"(m));
+  return m;
+}
+
+// a where the mask is set, else b, bit for bit.
+__device__ inline float blend(uint32_t m, float a, float b) {
+  return __uint_as_float((__float_as_uint(a) & m) | (__float_as_uint(b) & ~m));
+}
+
+// sum_i w_i c_i v_i (i = 1..18) for D3Q19: the unweighted sums of the two
+// weight classes, then one float-float product by w_1 (w_2 = w_1 / 2 exactly).
+template <class V>
+__device__ inline void weighted_c_sum(V v, float (&acc)[3]) {
+  using L = D3Q19;
+  float a1[3] = {0.0f, 0.0f, 0.0f}, a2[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int i = 1; i < 19; ++i) {
+    const float vi = v(i);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (L::c[i][d] == 0) continue;
+      float &a = i < 7 ? a1[d] : a2[d];
+      a = L::c[i][d] > 0 ? a + vi : a - vi;
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) acc[d] = ffmul(fmaf(0.5f, a2[d], a1[d]), kW1);
 }
 
 __device__ inline float shfl_xor32f(float v) { return __shfl_xor(v, 32); }
@@ -237,24 +280,18 @@ __device__ inline void shifted_moment(const float *n, float (&j)[3]) {
       if (L::c[i][d] != 0) j[d] += (float)L::c[i][d] * n[i];
 }
 
-// Shan-Chen and wall force (force.d:238-309), body force, velocity shift,
-// equilibrium and BGK relaxation of component f at one fluid site, in
-// deviation form.  cur: streamed deviations; rho_p, j_p: the partner
-// component's density and first moment; sw: the site's solid word;
-// psi(ff, i): psi_ff at x + c_i (i < 0: at x); wall_rho(i): rho_f at the solid
-// neighbour x + c_i; out(i, d): the new deviation of population i.  Every
-// schedule of shifted storage calls this, so they all give the same bits.
-template <bool SOLID, class Psi, class WallRho, class Out>
-__device__ inline void shifted_collide_site(int f, const float *cur, float rho_cur, float rho_p,
-                                            const float (&j)[3], const float (&j_p)[3],
-                                            uint32_t sw, const ShiftParams &p,
-                                            const ShiftLane &pl, Psi psi, WallRho wall_rho,
-                                            Out out) {
-  using L = D3Q19;
-  constexpr int Q = 19;
-  float F[3] = {0.0f, 0.0f, 0.0f};
-  const float psi1 = psi(f, -1);
+// Shan-Chen and wall force (force.d:238-309) and body force on component f
+// at one fluid site.  sw: the site's solid word; psi(ff, i): psi_ff at
+// x + c_i (i < 0: at x); wall_rho(i, b): rho_f at the solid neighbour x + c_i
+// when b, else any value read in bounds (the site's own).
+template <bool SOLID, class Psi, class WallRho>
+__device__ inline void shifted_force(int f, uint32_t sw, const ShiftParams &p,
+                                     const ShiftLane &pl, Psi psi, WallRho wall_rho,
+                                     float (&F)[3]) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) F[d] = 0.0f;
   if (p.sc) {
+    const float psi1 = psi(f, -1);
     // the other component first, then the own one: for the usual coupling
     // matrix (zero diagonal) both lanes of a site take the same branch
 #pragma unroll
@@ -263,38 +300,49 @@ __device__ inline void shifted_collide_site(int f, const float *cur, float rho_c
       const float cc = which ? pl.g_self : pl.g_other;
       if (cc == 0.0f) continue;
       const float psi_self = psi(f2, -1);
-      float acc[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-      for (int i = 1; i < Q; ++i) {
-        const bool sn = SOLID && ((sw >> i) & 1u);
-        const float psi_n = psi(f2, i);  // read before the test: no branch per read
-        const float psi2 = sn ? psi_self : psi_n;
-        const float wp = ffmul(psi2, d3q19_wff(i));
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-          if (L::c[i][d] != 0) acc[d] += (float)L::c[i][d] * wp;
-      }
+      float acc[3];
+      weighted_c_sum(
+          [&](int i) {
+            const float psi_n = psi(f2, i);
+            return SOLID ? blend(bit_mask(sw, i), psi_self, psi_n) : psi_n;
+          },
+          acc);
       const float pc = psi1 * cc;
 #pragma unroll
       for (int d = 0; d < 3; ++d) F[d] = fmaf(-pc, acc[d], F[d]);
     }
     const float wc = pl.wc_f;
-    if (SOLID && wc != 0.0f) {
+    // wall force: sum over the solid neighbours; skipped by waves none of
+    // whose sites has one
+    if (SOLID && wc != 0.0f && __any((sw & 0x7fffeu) != 0u)) {
+      float acc[3];
+      weighted_c_sum(
+          [&](int i) {
+            const uint32_t m = bit_mask(sw, i);
+            return blend(m, wall_rho(i, m != 0u), 0.0f);  // every lane loads
+          },
+          acc);
+      const float pw = psi1 * wc;
 #pragma unroll
-      for (int i = 0; i < Q; ++i) {
-        if ((sw >> i) & 1u) {
-          const float pf = ffmul(psi1, d3q19_wff(i)) * wall_rho(i) * wc;
-#pragma unroll
-          for (int d = 0; d < 3; ++d)
-            if (L::c[i][d] != 0) F[d] -= (float)L::c[i][d] * pf;
-        }
-      }
+      for (int d = 0; d < 3; ++d) F[d] = fmaf(-pw, acc[d], F[d]);
     }
   }
   if (p.has_body) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) F[d] += pl.body_f[d];
   }
+}
+
+// Velocity shift, equilibrium and BGK relaxation of component f at one fluid
+// site, in deviation form (force.d, collision.d, eqdist.d).  cur: streamed
+// deviations; rho_p, j_p: the partner component's density and first moment;
+// F: the force on f; out(i, d): the new deviation of population i.
+template <class Out>
+__device__ inline void shifted_relax(const float *cur, float rho_cur, float rho_p,
+                                     const float (&j)[3], const float (&j_p)[3],
+                                     const float (&F)[3], const ShiftParams &p,
+                                     const ShiftLane &pl, Out out) {
+  constexpr int Q = 19;
   const float rinv = 1.0f / rho_cur;
   float u[3];
   if (p.eq_form == kBDist2) {
@@ -308,16 +356,27 @@ __device__ inline void shifted_collide_site(int f, const float *cur, float rho_c
 #pragma unroll
     for (int d = 0; d < 3; ++d) u[d] = fmaf(pl.tau_f, fmaf(F[d], rinv, p.gacc[d]), j[d] * rinv);
   }
-  float e[Q];
-  eq_dev_d3q19(p.eq_form, u, e);
   // rho - rho0 summed from the deviations (not rho_cur - rho0, which would
   // bring back the rounding of rho_cur)
   const float drh = shifted_drho(cur);
+  float deq[Q];
+  eq_dev_d3q19(p.eq_form, u, rho_cur, drh, deq);
 #pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    const float deq = fmaf(rho_cur, e[i], ffmul(drh, d3q19_wff(i)));
-    out(i, fmaf(pl.om_f, deq - cur[i], cur[i]));
-  }
+  for (int i = 0; i < Q; ++i) out(i, fmaf(pl.om_f, deq[i] - cur[i], cur[i]));
+}
+
+// Force and relaxation of component f at one fluid site (the two above in
+// turn).  Every schedule of shifted storage goes through shifted_force and
+// shifted_relax, so they all give the same bits.
+template <bool SOLID, class Psi, class WallRho, class Out>
+__device__ inline void shifted_collide_site(int f, const float *cur, float rho_cur, float rho_p,
+                                            const float (&j)[3], const float (&j_p)[3],
+                                            uint32_t sw, const ShiftParams &p,
+                                            const ShiftLane &pl, Psi psi, WallRho wall_rho,
+                                            Out out) {
+  float F[3];
+  shifted_force<SOLID>(f, sw, p, pl, psi, wall_rho, F);
+  shifted_relax(cur, rho_cur, rho_p, j, j_p, F, p, pl, out);
 }
 
 }  // namespace dlbc

@@ -530,7 +530,6 @@ __device__ inline void ssplit_psi_block(const Store *__restrict__ A,
 template <class Store, bool SOLID, bool OFF32, bool STAGED = false>
 __device__ inline void ssplit_collide_block(const Store *__restrict__ A, Store *__restrict__ B,
                                             const uint32_t *__restrict__ nbw,
-                                            const double *__restrict__ S,
                                             const float *__restrict__ P, const Geo &g,
                                             const ShiftParams &p, int blk,
                                             float (*rows)[9][kShiftSites + 2]) {
@@ -563,12 +562,13 @@ __device__ inline void ssplit_collide_block(const Store *__restrict__ A, Store *
     if (staged) return staged_psi<SB>(rows, ff, i, k);
     return at<OFF32>(P + ff * g.sp, i < 0 ? s : nbr(i));
   };
-  shifted_collide_site<SOLID>(
-      0, n0, rho0, rho1, j0, j1, sw, p, pl0, psi,
-      [&](int i) { return (float)at<OFF32>(S, nbr(i)); }, [&](int i, float v) { n0[i] = v; });
-  shifted_collide_site<SOLID>(
-      1, n1, rho1, rho0, j1, j0, sw, p, pl1, psi,
-      [&](int i) { return (float)at<OFF32>(S + g.sp, nbr(i)); }, [&](int i, float v) {
+  // both forces first (P holds the density at solid sites: wall force), so
+  // the psi reads are consumed before the populations of a relaxation
+  float F0[3], F1[3];
+  shifted_force<SOLID>(0, sw, p, pl0, psi, [&](int i, bool) { return psi(0, i); }, F0);
+  shifted_force<SOLID>(1, sw, p, pl1, psi, [&](int i, bool) { return psi(1, i); }, F1);
+  shifted_relax(n0, rho0, rho1, j0, j1, F0, p, pl0, [&](int i, float v) { n0[i] = v; });
+  shifted_relax(n1, rho1, rho0, j1, j0, F1, p, pl1, [&](int i, float v) {
         pair_store_nt<Store>(elem<OFF32>(B + 2 * (int64_t)i * g.sp, 2 * (int64_t)s), n0[i], v);
       });
 }
@@ -580,22 +580,23 @@ __global__ void __launch_bounds__(kShiftThreads)
   ssplit_psi_block<Store, SOLID, OFF32>(A, nbw, P, g, p, split_block(g));
 }
 
+// At most 128 VGPRs: four waves per SIMD.
 template <class Store, bool SOLID, bool OFF32, bool STAGED>
-__global__ void __launch_bounds__(kShiftThreads)
+__global__ void __launch_bounds__(kShiftThreads) __attribute__((amdgpu_waves_per_eu(4)))
     k_ssplit_collide(const Store *__restrict__ A, Store *__restrict__ B,
-                     const uint32_t *__restrict__ nbw, const double *__restrict__ S,
-                     const float *__restrict__ P, Geo g, ShiftParams p) {
+                     const uint32_t *__restrict__ nbw, const float *__restrict__ P, Geo g,
+                     ShiftParams p) {
   __shared__ float rows[STAGED ? 2 : 1][9][kShiftSites + 2];
-  ssplit_collide_block<Store, SOLID, OFF32, STAGED>(A, B, nbw, S, P, g, p, split_block(g), rows);
+  ssplit_collide_block<Store, SOLID, OFF32, STAGED>(A, B, nbw, P, g, p, split_block(g), rows);
 }
 
 // Blocks of kShiftSites sites over a lattice.
 inline int shift_grid(int64_t sites) { return (int)((sites + kShiftSites - 1) / kShiftSites); }
 
 template <class Store, bool OFF32>
-inline void launch_ssplit_impl(const Store *a, Store *b, const uint32_t *nbw, const double *S,
-                               float *P, const Geo &g, const ShiftParams &p,
-                               hipStream_t stream, int pass) {
+inline void launch_ssplit_impl(const Store *a, Store *b, const uint32_t *nbw, float *P,
+                               const Geo &g, const ShiftParams &p, hipStream_t stream,
+                               int pass) {
   const int nb = shift_grid(g.sites);
   const dim3 grid(nb), block(kShiftThreads);
   if (pass == 0) {
@@ -609,29 +610,29 @@ inline void launch_ssplit_impl(const Store *a, Store *b, const uint32_t *nbw, co
     const bool staged = g.nz % kShiftSites == 0;
     if (p.has_solid && staged)
       hipLaunchKernelGGL((k_ssplit_collide<Store, true, OFF32, true>), grid, block, 0, stream, a,
-                         b, nbw, S, P, g, p);
+                         b, nbw, P, g, p);
     else if (p.has_solid)
       hipLaunchKernelGGL((k_ssplit_collide<Store, true, OFF32, false>), grid, block, 0, stream,
-                         a, b, nbw, S, P, g, p);
+                         a, b, nbw, P, g, p);
     else if (staged)
       hipLaunchKernelGGL((k_ssplit_collide<Store, false, OFF32, true>), grid, block, 0, stream,
-                         a, b, nbw, S, P, g, p);
+                         a, b, nbw, P, g, p);
     else
       hipLaunchKernelGGL((k_ssplit_collide<Store, false, OFF32, false>), grid, block, 0, stream,
-                         a, b, nbw, S, P, g, p);
+                         a, b, nbw, P, g, p);
   }
 }
 
 template <class Store>
-inline void launch_ssplit_d3q19(const void *A, void *B, const uint32_t *nbw, const double *S,
-                                float *P, const Geo &g, const ShiftParams &p,
-                                hipStream_t stream, int pass) {
+inline void launch_ssplit_d3q19(const void *A, void *B, const uint32_t *nbw, float *P,
+                                const Geo &g, const ShiftParams &p, hipStream_t stream,
+                                int pass) {
   const Store *a = static_cast<const Store *>(A);
   Store *b = static_cast<Store *>(B);
   if (split_off32(g))
-    launch_ssplit_impl<Store, true>(a, b, nbw, S, P, g, p, stream, pass);
+    launch_ssplit_impl<Store, true>(a, b, nbw, P, g, p, stream, pass);
   else
-    launch_ssplit_impl<Store, false>(a, b, nbw, S, P, g, p, stream, pass);
+    launch_ssplit_impl<Store, false>(a, b, nbw, P, g, p, stream, pass);
 }
 
 template <bool OFF32, bool STAGED>
