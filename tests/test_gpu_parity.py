@@ -152,14 +152,10 @@ def test_config2_spinodal_bitexact(D, oracle_mod, form):
     _same_state(g, o, "config2 " + form)
 
 
-@pytest.mark.parametrize("form", [
-    "SecondOrder",
-    pytest.param("BDist2", marks=pytest.mark.skipif(
-        not os.environ.get("DLBC_FULL_PARITY"),
-        reason="a second 70 s of oracle time; set DLBC_FULL_PARITY=1"))])
+@pytest.mark.parametrize("form", ["SecondOrder", "BDist2"])
 def test_config2_full_size_1000_steps_bitexact(D, oracle_mod, form):
     """The 128^3 spinodal workload at its full size and length, 1000 steps,
-    with the reference's default SecondOrder equilibrium (BDist2 on request):
+    with the reference's default SecondOrder equilibrium and with BDist2:
     bit-identical to the oracle.  (On MI355X the GPU takes 0.4-0.6 s
     including initialisation, the oracle 67-78 s on 16 cores.)"""
     p = _config2(128, form)
