@@ -415,14 +415,23 @@ __global__ void __launch_bounds__(kSplitThreads)
   double n[19];
   double rho = 0.0, mom[3] = {0.0, 0.0, 0.0};
   const bool bdist2 = p.eq_form == kBDist2;
-  if (!(sw >> 31)) {
+  // A solid site's lane copies its own populations (the reference copies them
+  // every step, advection.d:84-86): with every neighbour taken as solid, the
+  // pull loads population bounce(i) of the site itself into n[i], in the same
+  // load instructions as the fluid lanes' pulls, and the store below writes
+  // n[bounce(i)] back to population i.  Whole cache lines are then written;
+  // skipping the solid lanes left partially written lines, which cost 7 % of
+  // the pass on the 256^3 channel.
+  const bool solid_site = inside && (sw >> 31);
+  if (inside) {
     int x, y, z;
     lane_coords(s, g, x, y, z);
     o = nbr_offsets(g, x, y, z);
-    split_pull<SOLID, OFF32>(A + (int64_t)f * 19 * g.sp, g, s, o, sw, n);
+    split_pull<SOLID, OFF32>(A + (int64_t)f * 19 * g.sp, g, s, o, solid_site ? 0x7ffffu : sw,
+                             n);
   }
   if (staged) stage_psi_rows<SB, kSplitThreads, OFF32>(S, g, split_block(g) * SB, rows);
-  if (!(sw >> 31)) {
+  if (inside) {
     rho = density<L>(n);
     if (bdist2) first_moment<L>(n, mom);
   }
@@ -432,7 +441,9 @@ __global__ void __launch_bounds__(kSplitThreads)
     for (int d = 0; d < 3; ++d) xch[f][1 + d][k] = mom[d];
   }
   __syncthreads();
-  if (sw >> 31) return;  // solid populations stay as they are
+  if (!inside) return;
+  // (a solid lane runs the arithmetic below on its own populations, whose
+  // result it does not store: no divergent branch in a mixed wave)
   const double rho_partner = xch[f ^ 1][0][k];
   double mom_p[3];
   if (bdist2) {
@@ -461,7 +472,8 @@ __global__ void __launch_bounds__(kSplitThreads)
                             p, wv);
   double *__restrict__ Bf = B + (int64_t)f * 19 * g.sp;
   collide_d3q19_streamed(f, n, rho, total, F, wv, p, [&](int i, double v) {
-    __builtin_nontemporal_store(v, elem<OFF32>(Bf + (int64_t)i * g.sp, s));
+    __builtin_nontemporal_store(solid_site ? n[L::bounce[i]] : v,
+                                elem<OFF32>(Bf + (int64_t)i * g.sp, s));
   });
 }
 
@@ -541,17 +553,21 @@ __device__ inline void ssplit_collide_block(const Store *__restrict__ A, Store *
   const uint32_t sw = s >= g.sites ? (1u << 31) : (SOLID ? nbw[s] : 0u);
   NbrOffsets o{};
   float n0[19], n1[19];
-  if (!(sw >> 31)) {
+  // a solid site's lane copies its own populations, as in the FP64 pass: the
+  // pull with every neighbour taken as solid loads population bounce(i) of the
+  // site into n[i], and the store writes n[bounce(i)] back to population i
+  const bool inside = s < g.sites, solid_site = inside && (sw >> 31);
+  if (inside) {
     int x, y, z;
     lane_coords(s, g, x, y, z);
     o = nbr_offsets(g, x, y, z);
-    ssplit_pull<Store, SOLID, OFF32>(A, g, s, o, sw, n0, n1);
+    ssplit_pull<Store, SOLID, OFF32>(A, g, s, o, solid_site ? 0x7ffffu : sw, n0, n1);
   }
   if (staged) {
     stage_psi_rows<SB, kShiftThreads, OFF32>(P, g, blk * SB, rows);
     __syncthreads();
   }
-  if (sw >> 31) return;  // solid populations stay as they are
+  if (!inside) return;
   const ShiftLane pl0 = shift_lane(p, 0), pl1 = shift_lane(p, 1);
   const float rho0 = pl0.rho0 + shifted_drho(n0), rho1 = pl1.rho0 + shifted_drho(n1);
   float j0[3], j1[3];
@@ -567,10 +583,13 @@ __device__ inline void ssplit_collide_block(const Store *__restrict__ A, Store *
   float F0[3], F1[3];
   shifted_force<SOLID>(0, sw, p, pl0, psi, [&](int i, bool) { return psi(0, i); }, F0);
   shifted_force<SOLID>(1, sw, p, pl1, psi, [&](int i, bool) { return psi(1, i); }, F1);
-  shifted_relax(n0, rho0, rho1, j0, j1, F0, p, pl0, [&](int i, float v) { n0[i] = v; });
+  shifted_relax(n0, rho0, rho1, j0, j1, F0, p, pl0,
+                [&](int i, float v) { n0[i] = solid_site ? n0[i] : v; });
   shifted_relax(n1, rho1, rho0, j1, j0, F1, p, pl1, [&](int i, float v) {
-        pair_store_nt<Store>(elem<OFF32>(B + 2 * (int64_t)i * g.sp, 2 * (int64_t)s), n0[i], v);
-      });
+    const int b = L::bounce[i];
+    pair_store_nt<Store>(elem<OFF32>(B + 2 * (int64_t)i * g.sp, 2 * (int64_t)s),
+                         solid_site ? n0[b] : n0[i], solid_site ? n1[b] : v);
+  });
 }
 
 template <class Store, bool SOLID, bool OFF32>
