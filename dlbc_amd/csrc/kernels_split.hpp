@@ -70,6 +70,18 @@ constexpr bool kSplitAutoChunk = false;
 // order_main for chunk K over a grid of nb blocks: whole turns of 8 K blocks.
 inline int split_order_main(int nb, int K) { return K > 0 ? nb / (8 * K) * (8 * K) : 0; }
 
+// Whether every site of this lane's cache line is solid, for a wave of 64
+// consecutive sites whose lines hold LINE sites each (lines aligned to the
+// wave: the site stride is a multiple of 64).  Such lines are neither read nor
+// written by a collide pass: a solid site keeps its populations.
+template <int LINE>
+__device__ inline bool line_all_solid(bool solid_site) {
+  constexpr uint64_t kLineMask = LINE == 64 ? ~0ull : ((1ull << LINE) - 1ull);
+  const uint64_t m = __ballot(solid_site);
+  const int first = (threadIdx.x & 63) / LINE * LINE;
+  return ((m >> first) & kLineMask) == kLineMask;
+}
+
 // Wave-uniform copy of an int: the compiler then keeps it in a scalar
 // register, and pointers formed from it stay scalar (the wave index
 // threadIdx.x >> 6 is uniform, but the compiler cannot tell).
@@ -423,7 +435,8 @@ __global__ void __launch_bounds__(kSplitThreads)
   // skipping the solid lanes left partially written lines, which cost 7 % of
   // the pass on the 256^3 channel.
   const bool solid_site = inside && (sw >> 31);
-  if (inside) {
+  const bool active = inside && !(SOLID && line_all_solid<16>(solid_site));
+  if (active) {
     int x, y, z;
     lane_coords(s, g, x, y, z);
     o = nbr_offsets(g, x, y, z);
@@ -431,7 +444,7 @@ __global__ void __launch_bounds__(kSplitThreads)
                              n);
   }
   if (staged) stage_psi_rows<SB, kSplitThreads, OFF32>(S, g, split_block(g) * SB, rows);
-  if (inside) {
+  if (active) {
     rho = density<L>(n);
     if (bdist2) first_moment<L>(n, mom);
   }
@@ -441,7 +454,7 @@ __global__ void __launch_bounds__(kSplitThreads)
     for (int d = 0; d < 3; ++d) xch[f][1 + d][k] = mom[d];
   }
   __syncthreads();
-  if (!inside) return;
+  if (!active) return;
   // (a solid lane runs the arithmetic below on its own populations, whose
   // result it does not store: no divergent branch in a mixed wave)
   const double rho_partner = xch[f ^ 1][0][k];
@@ -557,7 +570,10 @@ __device__ inline void ssplit_collide_block(const Store *__restrict__ A, Store *
   // pull with every neighbour taken as solid loads population bounce(i) of the
   // site into n[i], and the store writes n[bounce(i)] back to population i
   const bool inside = s < g.sites, solid_site = inside && (sw >> 31);
-  if (inside) {
+  // a 128-byte line holds 16 (FP32) or 32 (FP16) sites' pairs
+  const bool active =
+      inside && !(SOLID && line_all_solid<128 / (2 * (int)sizeof(Store))>(solid_site));
+  if (active) {
     int x, y, z;
     lane_coords(s, g, x, y, z);
     o = nbr_offsets(g, x, y, z);
@@ -567,7 +583,7 @@ __device__ inline void ssplit_collide_block(const Store *__restrict__ A, Store *
     stage_psi_rows<SB, kShiftThreads, OFF32>(P, g, blk * SB, rows);
     __syncthreads();
   }
-  if (!inside) return;
+  if (!active) return;
   const ShiftLane pl0 = shift_lane(p, 0), pl1 = shift_lane(p, 1);
   const float rho0 = pl0.rho0 + shifted_drho(n0), rho1 = pl1.rho0 + shifted_drho(n1);
   float j0[3], j1[3];
