@@ -195,39 +195,43 @@ __device__ inline void lane_coords(int s, const Geo &g, int &x, int &y, int &z) 
 // multiple of SB), staged in LDS: the nine rows (x + dx, y + dy), dx, dy in
 // {-1, 0, 1}, from z0 - 1 to z0 + SB (periodic), of both components.  Every
 // neighbour psi (and the solid density in the same array) of the block is in
-// them.  Each wave takes whole (component, row) pairs: a wave-uniform row base
-// and consecutive z per lane, no wrap inside the block's span; the 36 end
-// values (z0 - 1 and z0 + SB of each pair) take one pass of wave 0.  The
-// loads go out first and the LDS stores follow, so that the loads overlap
-// the population pulls issued before; the caller's barrier publishes the
-// rows.
+// them.  The rows are read with 16-byte loads (four FP32 or two FP64 values
+// per lane, aligned: n_z, z0 and the component stride are multiples of 64),
+// in units of one row and one 64-lane pass, dealt round-robin to the waves:
+// a wave-uniform row base and consecutive z per lane, no wrap inside the
+// block's span.  The 36 end values (z0 - 1 and z0 + SB of each row) take one
+// pass of wave 0.  The loads go out first and the LDS stores follow, so that
+// the loads overlap the population pulls issued before; the caller's barrier
+// publishes the rows.
 template <int SB, int THREADS, bool OFF32, class T>
 __device__ inline void stage_psi_rows(const T *__restrict__ S, const Geo &g, int s0,
                                       T (*rows)[9][SB + 2]) {
-  constexpr int NW = THREADS / 64, NP = 18, PASS = SB / 64, PPW = (NP + NW - 1) / NW;
-  static_assert(SB % 64 == 0, "staged rows take whole lane passes");
+  constexpr int VEC = 16 / (int)sizeof(T);  // values per 16-byte load
+  typedef T Vec __attribute__((ext_vector_type(VEC)));
+  constexpr int NW = THREADS / 64, NR = 18, PASS = SB / (64 * VEC);
+  constexpr int NU = NR * PASS, UPW = (NU + NW - 1) / NW;  // units, per wave
+  static_assert(SB % (64 * VEC) == 0, "staged rows take whole lane passes");
   int x0, y0, z0;
   coords(uniform(s0), g, x0, y0, z0);
   const int w = uniform(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  // element offset of pair pr's row: component pr / 9, row (dx, dy) = pr % 9
-  // (the wrap is arithmetic: a select among three coordinates became a
-  // lookup table in scratch memory)
+  // element offset of row pr: component pr / 9, row (dx, dy) = pr % 9 (the
+  // wrap is arithmetic: a select among three coordinates became a lookup
+  // table in scratch memory)
   auto pair_base = [&](int pr) {
     const int row = pr % 9;
     const int xx = wrap(x0 + row / 3 - 1, g.nx), yy = wrap(y0 + row % 3 - 1, g.ny);
     return (int64_t)(pr / 9) * g.sp + (int64_t)(xx * g.ny + yy) * g.nz;
   };
-  T v[PPW * PASS];
+  Vec v[UPW];
 #pragma unroll
-  for (int q = 0; q < PPW; ++q) {
-    const int pr = w + q * NW;
-    const T *base = S + pair_base(pr < NP ? pr : 0) + z0;
-#pragma unroll
-    for (int k = 0; k < PASS; ++k)
-      v[q * PASS + k] = pr < NP ? at<OFF32>(base, lane + 64 * k) : T(0);
+  for (int q = 0; q < UPW; ++q) {
+    const int u = w + q * NW, pr = (u < NU ? u : 0) / PASS, k = (u < NU ? u : 0) % PASS;
+    const T *base = S + pair_base(pr) + z0;
+    v[q] = u < NU ? *reinterpret_cast<const Vec *>(elem<OFF32>(base, VEC * (lane + 64 * k)))
+                  : Vec(T(0));
   }
-  // the two ends of every pair's span: lane 2 pr + side of wave 0
-  const bool edge = w == 0 && lane < 2 * NP;
+  // the two ends of every row's span: lane 2 pr + side of wave 0
+  const bool edge = w == 0 && lane < 2 * NR;
   const int epr = lane >> 1, side = lane & 1;
   T ve = T(0);
   if (edge) {
@@ -235,12 +239,12 @@ __device__ inline void stage_psi_rows(const T *__restrict__ S, const Geo &g, int
     ve = at<OFF32>(S, pair_base(epr) + ze);
   }
 #pragma unroll
-  for (int q = 0; q < PPW; ++q) {
-    const int pr = w + q * NW;
-    if (pr < NP) {
-      T *row = rows[pr / 9][pr % 9] + 1 + lane;
+  for (int q = 0; q < UPW; ++q) {
+    const int u = w + q * NW;
+    if (u < NU) {
+      T *row = rows[u / PASS / 9][u / PASS % 9] + 1 + VEC * (lane + 64 * (u % PASS));
 #pragma unroll
-      for (int k = 0; k < PASS; ++k) row[64 * k] = v[q * PASS + k];
+      for (int j = 0; j < VEC; ++j) row[j] = v[q][j];
     }
   }
   if (edge) rows[epr / 9][epr % 9][side ? SB + 1 : 0] = ve;
