@@ -1014,6 +1014,7 @@ int dlbc_gpu_set_params(dlbc_gpu_ctx *c, const double *tau, const double *gcc, c
     ShiftParams sp{};
     for (int f = 0; f < 2; ++f) {
       sp.tau[f] = (float)p.tau[f];
+      sp.itau[f] = 1.0f / sp.tau[f];
       sp.omega[f] = (float)p.omega[f];
       sp.rho0[f] = (float)c->rho0[f];
       sp.gwc[f] = (float)p.gwc[f];

@@ -118,7 +118,7 @@ __device__ inline void pair_store_nt<__half>(__half *p, float a, float b) {
 }
 
 struct ShiftParams {
-  float tau[2], omega[2], rho0[2];
+  float tau[2], itau[2], omega[2], rho0[2];  // itau = 1.0f / tau, rounded once on the host
   float gcc[2][2], gwc[2];
   float gacc[3], body[2][3];
   int eq_form, psi_form, sc, has_solid, has_body;
@@ -258,8 +258,8 @@ __device__ inline ShiftLane shift_lane(const ShiftParams &p, int f) {
   ShiftLane l;
   l.rho0 = sel(p.rho0[0], p.rho0[1]);
   l.tau_f = sel(p.tau[0], p.tau[1]);
-  l.itau_f = 1.0f / l.tau_f;
-  l.itau_o = 1.0f / sel(p.tau[1], p.tau[0]);
+  l.itau_f = sel(p.itau[0], p.itau[1]);
+  l.itau_o = sel(p.itau[1], p.itau[0]);
   l.om_f = sel(p.omega[0], p.omega[1]);
   l.g_self = sel(p.gcc[0][0], p.gcc[1][1]);
   l.g_other = sel(p.gcc[0][1], p.gcc[1][0]);
