@@ -189,10 +189,12 @@ __global__ void k_fill_default(St *__restrict__ A, St *__restrict__ B, int64_t n
   }
 }
 
-// The step kernels never write a solid site: its populations stay as they are,
-// as in the reference (advection.d:84-86), provided both buffers hold the same
-// values there.  After a mask change mid-run, copy the current buffer A into B
-// at every solid site.
+// A solid site's populations stay as they are, as in the reference
+// (advection.d:84-86): the step kernels either skip it or (the split collide
+// passes, where its cache line also holds fluid sites) store its own current
+// populations again.  Skipping is right only while both buffers hold the same
+// values there, so after a mask change mid-run the current buffer A is copied
+// into B at every solid site.
 template <class St>
 __global__ void k_copy_solid(const St *__restrict__ A, St *__restrict__ B,
                              const uint8_t *__restrict__ mask, Geo g, int C, int q) {
