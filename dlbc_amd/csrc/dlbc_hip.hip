@@ -919,6 +919,7 @@ int dlbc_gpu_create(dlbc_gpu_ctx **out, int d, int q, int components, const int6
   c->geo.nz = c->n[2];
   c->geo.sites = (int)sites;
   c->geo.sp = (sites + 63) / 64 * 64;
+  geo_set_divisors(c->geo);
   apply_block_order(c, -1);
   auto bail = [&](int code) {
     dlbc_gpu_destroy(c);

@@ -112,8 +112,12 @@ __device__ inline void pair_store_nt<float>(float *p, float a, float b) {
 }
 template <>
 __device__ inline void pair_store_nt<__half>(__half *p, float a, float b) {
-  const uint32_t v = ((uint32_t)__half_as_ushort(__float2half_rn(materialise(b))) << 16) |
-                     __half_as_ushort(__float2half_rn(materialise(a)));
+  // both conversions in one packed instruction, rounded to nearest even as
+  // __float2half_rn rounds
+  typedef float F2 __attribute__((ext_vector_type(2)));
+  typedef _Float16 H2 __attribute__((ext_vector_type(2)));
+  const F2 in = {materialise(a), materialise(b)};
+  const uint32_t v = __builtin_bit_cast(uint32_t, __builtin_convertvector(in, H2));
   __builtin_nontemporal_store(v, reinterpret_cast<uint32_t *>(p));
 }
 

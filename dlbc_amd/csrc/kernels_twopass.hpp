@@ -27,7 +27,32 @@ struct Geo {
   // K > 0 hands the XCDs K consecutive blocks each in turn over the first
   // order_main blocks; 0 keeps site order.
   int order_chunk = 0, order_main = 0;
+  // s / (n_y n_z) and s / n_z as a multiplication (coords): q = (s m) >> (32 + l)
+  // with 2^l >= d and m = ceil(2^(32 + l) / d), which is exact for every
+  // s < 2^32 (m d - 2^(32 + l) < d, so s times that excess stays below
+  // 2^(32 + l)); a 32-bit division costs some 30 instructions per lane.
+  uint64_t div_yz_m = 0, div_z_m = 0;
+  int div_yz_l = 0, div_z_l = 0;
 };
+
+// The multiplier and shift of division by d (1 <= d <= 2^30) for Geo.
+__host__ inline void geo_divisor(uint32_t d, uint64_t &m, int &l) {
+  l = 0;
+  while ((1ull << l) < d) ++l;
+  m = ((1ull << (32 + l)) + d - 1) / d;
+}
+
+// Geo's divisors from its extents (after n_x, n_y, n_z are set).
+__host__ inline void geo_set_divisors(Geo &g) {
+  geo_divisor((uint32_t)g.ny * (uint32_t)g.nz, g.div_yz_m, g.div_yz_l);
+  geo_divisor((uint32_t)g.nz, g.div_z_m, g.div_z_l);
+}
+
+// floor(s / d) for 0 <= s < 2^31 by Geo's multiplier and shift for d
+// (s m < 2^64 since m < 2^33).
+__device__ inline int geo_div(int s, uint64_t m, int l) {
+  return (int)(((uint64_t)(uint32_t)s * m) >> (32 + l));
+}
 
 struct StepParams {
   double tau[4], omega[4], one_minus_omega[4];
@@ -40,9 +65,9 @@ struct StepParams {
 
 __device__ inline void coords(int s, const Geo &g, int &x, int &y, int &z) {
   const int yz = g.ny * g.nz;
-  x = s / yz;
+  x = geo_div(s, g.div_yz_m, g.div_yz_l);
   const int r = s - x * yz;
-  y = r / g.nz;
+  y = geo_div(r, g.div_z_m, g.div_z_l);
   z = r - y * g.nz;
 }
 
