@@ -168,6 +168,28 @@ def test_shifted_split_schedule_equals_tiled(D, storage, walls, n):
         assert np.array_equal(a.population(f).view(np.uint64), b.population(f).view(np.uint64))
 
 
+@pytest.mark.parametrize("storage", ["f32_shift", "f16_shift"])
+def test_shifted_split_whole_solid_lines_equal_tiled(D, storage):
+    """Solid boxes covering whole cache lines along z (16 FP32 or 32 FP16
+    site pairs a line), which the split collide pass neither reads nor writes,
+    and lines of which only part is solid, whose solid sites it copies: the
+    same bits as the tiled kernel, which never writes a solid site."""
+    n = (6, 7, 256)
+    p = droplet(n, 1.2, 3.8, 0.3)
+    mask = np.zeros(n, dtype=np.int32)
+    mask[:, :, 0] = 1
+    mask[:, :, -1] = 1
+    mask[1:3, 2:5, 64:128] = 1
+    mask[4, :, 150:157] = 1
+    p = dict(p, maskArray=mask, gwc=[0.2, -0.1])
+    a = D.Lattice.from_params(p, storage=storage, schedule="fused")
+    b = D.Lattice.from_params(p, storage=storage, schedule="split")
+    a.step(25)
+    b.step(25)
+    for f in range(2):
+        assert np.array_equal(a.population(f).view(np.uint64), b.population(f).view(np.uint64))
+
+
 def test_config5_ensemble_16_lattices(D):
     """The batched-ensemble workload cut to its first 16 lattices (L from 32
     to 96, tau, g and radius per lattice from the workload's seed), FP32

@@ -428,6 +428,29 @@ def test_split_schedule_matches_oracle(D, oracle_mod, form, walls, n):
     _same_state(g, o, "split %s walls=%s" % (form, walls))
 
 
+def _solid_blocks_case(n, form="BDist2"):
+    """Walls case plus solid boxes that cover whole cache lines along z (16
+    FP64 sites a line) and lines of which only part is solid."""
+    p = _walls_case(n, form)
+    mask = p["maskArray"].copy()
+    mask[1:4, 2:6, 16:48] = 1   # two whole 16-site lines per row
+    mask[5:7, :, 70:75] = 1     # parts of lines
+    return dict(p, maskArray=mask)
+
+
+@pytest.mark.parametrize("n", [[8, 9, 128], [6, 7, 100]], ids=["z-rows", "unaligned"])
+def test_split_whole_solid_lines_bitexact(D, oracle_mod, n):
+    """The split collide pass leaves cache lines of solid sites only alone and
+    copies the populations of the solid sites of mixed lines: after 30 steps
+    every population, solid sites' included, equals the oracle's bit for bit."""
+    p = _solid_blocks_case(n)
+    g = D.Lattice.from_params(p, schedule="split")
+    o = oracle_mod.Oracle.from_params(p)
+    g.step(30)
+    o.run(30)
+    _same_state(g, o, "split, solid boxes")
+
+
 # (z-rows per workgroup, populations held in registers, launch target):
 # one row pulling both neighbouring rows, the same holding its populations,
 # two rows, and the two row counts with one chunk of x per workgroup.
