@@ -190,6 +190,38 @@ def test_shifted_split_whole_solid_lines_equal_tiled(D, storage):
         assert np.array_equal(a.population(f).view(np.uint64), b.population(f).view(np.uint64))
 
 
+@pytest.mark.parametrize("form,psi,gself", [("SecondOrder", "Linear", 0.0),
+                                             ("ThirdOrder", "Exponential", 0.4),
+                                             ("BDist2", "Linear", 0.3)],
+                         ids=["second-linear", "third-exp-self", "bdist2-linear-self"])
+@pytest.mark.parametrize("storage", ["f32_shift", "f16_shift"])
+def test_shifted_forms_split_equals_tiled(D, storage, form, psi, gself):
+    """The other equilibria (SecondOrder, ThirdOrder: the weights' float-float
+    products on rho t + (rho - rho0)), psi = rho, and self-couplings g_ff != 0
+    (the second Shan-Chen loop) under the split and tiled schedules: the same
+    bits, with walls, random solids and a wall coupling; FP32 also stays within
+    2e-4 of the FP64 lattice in density."""
+    n = (8, 9, 256)
+    rng = np.random.default_rng(9)
+    mask = (rng.random(n) < 0.08).astype(np.int32)
+    mask[:, :, 0] = 1
+    p = dict(droplet(n, 1.1, 2.5, 0.3), eqDistForm=form, psiForm=psi,
+             gcc=[[gself, 2.5], [2.5, 0.5 * gself]], maskArray=mask, gwc=[0.1, -0.05])
+    a = D.Lattice.from_params(p, storage=storage, schedule="fused")
+    b = D.Lattice.from_params(p, storage=storage, schedule="split")
+    a.step(20)
+    b.step(20)
+    for f in range(2):
+        assert np.array_equal(a.population(f).view(np.uint64), b.population(f).view(np.uint64))
+    if storage == "f32_shift":
+        c = D.Lattice.from_params(p)
+        c.step(20)
+        fluid = mask == 0
+        for f in range(2):
+            d = np.abs(b.density(f)[fluid] - c.density(f)[fluid]).max()
+            assert d < 2e-4, (f, d)
+
+
 def test_config5_ensemble_16_lattices(D):
     """The batched-ensemble workload cut to its first 16 lattices (L from 32
     to 96, tau, g and radius per lattice from the workload's seed), FP32
