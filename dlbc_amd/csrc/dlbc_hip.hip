@@ -1641,6 +1641,34 @@ int dlbc_gpu_eval_psi(dlbc_gpu_ctx *c, int psi_form, const double *x, double *y,
   return DLBC_OK;
 }
 
+int64_t dlbc_selftest_site_division(int64_t n, uint64_t seed) {
+  uint64_t st = seed * 0x9E3779B97F4A7C15ull + 7;
+  auto next = [&]() {  // splitmix64
+    uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  };
+  int64_t bad = 0;
+  for (int64_t k = 0; k < n; ++k) {
+    const uint64_t r = next();
+    // divisors: every n_z or n_y n_z of a lattice (1 .. 2^30), small ones often
+    uint32_t d = (k & 1) ? 1u + (uint32_t)(r % 4096u) : 1u + (uint32_t)(r % (1u << 30));
+    uint64_t m;
+    int l;
+    geo_divisor(d, m, l);
+    // dividends: site indices below 2^31, multiples of d and their neighbours often
+    const uint64_t r2 = next();
+    int64_t sv = (int64_t)(r2 % (1ull << 31));
+    if ((k >> 1) & 1) sv = (sv / d) * d - 1 + (int64_t)(r2 >> 62);
+    if (sv < 0) sv = 0;
+    if (sv > 0x7fffffff) sv = 0x7fffffff;
+    const int s = (int)sv;
+    if (geo_div(s, m, l) != s / (int)d) ++bad;
+  }
+  return bad;
+}
+
 const char *dlbc_gpu_last_error(const dlbc_gpu_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
 int64_t dlbc_gpu_device_bytes(const dlbc_gpu_ctx *c) { return c ? c->dev_bytes : 0; }

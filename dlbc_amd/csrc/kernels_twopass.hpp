@@ -50,7 +50,7 @@ __host__ inline void geo_set_divisors(Geo &g) {
 
 // floor(s / d) for 0 <= s < 2^31 by Geo's multiplier and shift for d
 // (s m < 2^64 since m < 2^33).
-__device__ inline int geo_div(int s, uint64_t m, int l) {
+__host__ __device__ inline int geo_div(int s, uint64_t m, int l) {
   return (int)(((uint64_t)(uint32_t)s * m) >> (32 + l));
 }
 

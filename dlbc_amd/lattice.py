@@ -87,6 +87,7 @@ def lib():
         "dlbc_gpu_eval_psi": (i, [vp, i, _dp, _dp, i64]),
         "dlbc_gpu_last_error": (C.c_char_p, [vp]),
         "dlbc_selftest_division": (i64, [i64, C.c_uint64]),
+        "dlbc_selftest_site_division": (i64, [i64, C.c_uint64]),
         "dlbc_gpu_destroy": (None, [vp]),
         "dlbc_gpu_device_bytes": (i64, [vp]),
         "dlbc_rng_seed": (None, [C.POINTER(_Rng), C.c_uint32]),

@@ -220,6 +220,14 @@ int dlbc_gpu_eval_psi(dlbc_gpu_ctx *ctx, int psi_form, const double *x, double *
  * (0 expected).  Host only, no GPU needed. */
 int64_t dlbc_selftest_division(int64_t n, uint64_t seed);
 
+/* Self-check of the site-coordinate division the kernels use (a multiply and
+ * a shift by per-lattice constants in place of 32-bit integer division):
+ * compares it with integer division on n pseudo-random (dividend, divisor)
+ * pairs over the index range of a lattice, half of them next to exact
+ * multiples, and returns the number of mismatches (0 expected).  Host only,
+ * no GPU needed; no counterpart in the reference. */
+int64_t dlbc_selftest_site_division(int64_t n, uint64_t seed);
+
 const char *dlbc_gpu_last_error(const dlbc_gpu_ctx *ctx);
 void dlbc_gpu_destroy(dlbc_gpu_ctx *ctx);
 

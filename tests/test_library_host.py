@@ -137,6 +137,15 @@ def test_exact_constant_division_selftest():
     assert D.lib().dlbc_selftest_division(4_000_000, 2026) == 0
 
 
+def test_site_division_selftest():
+    """Site coordinates come from a multiply and a shift by per-lattice
+    constants (m = ceil(2^(32+l) / d), 2^l >= d) instead of integer division;
+    on 4 million (site, divisor) pairs over the range of lattice indices, half
+    of them next to exact multiples, it equals integer division."""
+    import dlbc_amd as D
+    assert D.lib().dlbc_selftest_site_division(4_000_000, 2027) == 0
+
+
 @pytest.mark.parametrize("d,q,nc,n", [(2, 9, 2, [64, 64]), (3, 19, 1, [8, 8, 32]),
                                       (3, 19, 2, [8, 3, 32]), (3, 19, 2, [8, 8, 2])])
 def test_shifted_storage_shape_checks(d, q, nc, n):
