@@ -53,3 +53,28 @@ def test_random_lattices_bitexact(D, oracle_mod, seed):
     for f in range(2):
         assert np.array_equal(g.population(f).view(np.uint64),
                               o.population(f).view(np.uint64)), (p["gn"], p["eqDistForm"], f)
+
+
+def _shifted_case(seed):
+    """A random lattice for the shifted storages (n >= 4 on every axis)."""
+    p, steps = _case(seed)
+    rng = np.random.default_rng(seed + 7)
+    n = [max(4, v) for v in p["gn"]]
+    mask = (rng.random(n) < float(rng.choice([0.0, 0.1, 0.3]))).astype(np.int32)
+    return dict(p, gn=n, maskArray=mask, rho0=[0.5, 0.5]), steps
+
+
+@pytest.mark.parametrize("seed", range(200, 210))
+@pytest.mark.parametrize("storage", ["f32_shift", "f16_shift"])
+def test_random_lattices_shifted_split_equals_tiled(D, storage, seed):
+    """The same random lattices in FP32- and FP16-shifted storage: the split
+    passes (the default) and the tiled kernel give the same bits."""
+    p, steps = _shifted_case(seed)
+    a = D.Lattice.from_params(p, storage=storage)
+    b = D.Lattice.from_params(p, storage=storage, schedule="fused")
+    assert a.schedule == "split"
+    a.step(steps)
+    b.step(steps)
+    for f in range(2):
+        assert np.array_equal(a.population(f).view(np.uint64),
+                              b.population(f).view(np.uint64)), (p["gn"], p["eqDistForm"], f)
