@@ -685,6 +685,27 @@ def test_bench_channel_workload_bitexact_at_64(D, oracle_mod):
     _same_state(g, o, "channel 64^3")
 
 
+def test_bench_channel_workload_full_size_bitexact(D, oracle_mod):
+    """The bench.py channel workload at its full 256^3 (16.8 M sites, solid
+    fraction 0.2, body force on component 1), 30 steps on the default
+    schedule: bit-identical to the oracle (about 20 s of oracle time and
+    about 20 GB of host memory for the two states)."""
+    import bench
+    n = 256
+    p = bench.channel_params(n)
+    p["maskArray"] = bench.channel_mask(n)
+    g, o = _pair(D, oracle_mod, p)
+    assert g.schedule == "split"
+    g.step(30)
+    o.run(30)
+    for f in range(2):
+        a, b = g.population(f), o.population(f)
+        assert a.shape == b.shape
+        bad = int(np.count_nonzero(a.view(np.uint64) != b.view(np.uint64)))
+        assert bad == 0, "channel 256^3 population %d: %d values differ" % (f, bad)
+        del a, b
+
+
 @pytest.mark.parametrize("name", ["checkpoint_2d", "checkpoint_d1q3"])
 def test_checkpoint_lowdim_restart(D, oracle_mod, name):
     """technical/checkpoint-2d (D2Q9) and checkpoint-1d (D1Q3): the t0 restart
