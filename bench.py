@@ -200,10 +200,38 @@ def storage_leg(D, p, mask, pops, storage, steps, warmup, fluid):
     return out
 
 
+def interface_displacement(phi, phi_ref, fluid):
+    """Displacement of the colour interface (phi = 0) from its position in a
+    reference field, in lattice units, to first order: at every fluid site
+    where the reference colour changes sign towards a fluid face neighbour,
+    |phi - phi_ref| over the reference colour gradient there (central
+    differences, taken only where both neighbours along every axis are
+    fluid).  Returns the largest and the mean displacement, the number of
+    interface sites they cover, and the number of fluid sites whose colour
+    has the other sign than in the reference."""
+    iface = np.zeros(phi_ref.shape, bool)
+    inner = fluid.copy()
+    grad2 = np.zeros(phi_ref.shape)
+    sref = np.sign(phi_ref)
+    for ax in range(phi_ref.ndim):
+        up, dn = np.roll(phi_ref, -1, ax), np.roll(phi_ref, 1, ax)
+        fu, fd = np.roll(fluid, -1, ax), np.roll(fluid, 1, ax)
+        inner &= fu & fd
+        iface |= fluid & ((fu & (np.sign(up) != sref)) | (fd & (np.sign(dn) != sref)))
+        grad2 += (0.5 * (up - dn)) ** 2
+    sel = iface & inner
+    disp = np.abs(phi[sel] - phi_ref[sel]) / np.maximum(np.sqrt(grad2[sel]), 1e-300)
+    flips = int(np.count_nonzero(fluid & (np.sign(phi) != sref)))
+    if disp.size == 0:
+        return 0.0, 0.0, 0, flips
+    return float(disp.max()), float(disp.mean()), int(disp.size), flips
+
+
 def storage_accuracy(D, p, mask, pops, storages, steps):
     """Error of the shifted storages against the FP64 path on the same
     workload after `steps` steps: density, velocity (max norm relative to the
-    largest FP64 speed, and relative 2-norm over all fluid sites) and mass."""
+    largest FP64 speed, and relative 2-norm over all fluid sites), mass and
+    the position of the red/blue interface (interface_displacement)."""
     import torch
     fluid = mask == 0
 
@@ -214,6 +242,7 @@ def storage_accuracy(D, p, mask, pops, storages, steps):
         L.step(steps)
         out = {"rho": [L.density(f) for f in range(2)],
                "u": [L.velocity(f) for f in range(2)],
+               "colour": L.colour(),
                "drift": max(abs(L.total_mass(f) - m0[f]) / m0[f] for f in range(2))}
         del L
         torch.cuda.empty_cache()
@@ -236,6 +265,9 @@ def storage_accuracy(D, p, mask, pops, storages, steps):
                              for f in range(2)),
             "mass_rel_drift": r["drift"], "f64_mass_rel_drift": ref["drift"],
             "rho0": rho0_for(st)}
+        dmax, dmean, nif, flips = interface_displacement(r["colour"], ref["colour"], fluid)
+        res[st].update({"interface_disp_max": dmax, "interface_disp_mean": dmean,
+                        "interface_sites": nif, "colour_sign_flips": flips})
         log("accuracy %s after %d steps: %s" % (st, steps, res[st]))
     return res
 
@@ -299,7 +331,8 @@ def droplet_params(n, radius):
 def droplet_workload(D, n, steps, storages=("f16_shift", "f32_shift")):
     """Shifted storages against FP64 on the droplet: throughput of each, and
     after `steps` steps the largest density and velocity differences, the
-    droplet radius of the Laplace diagnostics and the mass drift."""
+    droplet radius of the Laplace diagnostics, the interface displacement
+    (interface_displacement) and the mass drift."""
     import torch
     radius = 96.0 * n / 512.0
     p = droplet_params(n, radius)
@@ -348,7 +381,8 @@ def droplet_workload(D, n, steps, storages=("f16_shift", "f32_shift")):
         return L, out
 
     ref, res = run("f64")
-    fields = {"rho": [ref.density(f) for f in range(2)], "u": [ref.velocity(f) for f in range(2)]}
+    fields = {"rho": [ref.density(f) for f in range(2)], "u": [ref.velocity(f) for f in range(2)],
+              "colour": ref.colour()}
     result = {"f64": res}
     del ref
     torch.cuda.empty_cache()
@@ -362,6 +396,10 @@ def droplet_workload(D, n, steps, storages=("f16_shift", "f32_shift")):
         res["max_abs_du_over_max_u"] = res["max_abs_du"] / umax
         res["radius_rel_diff"] = (res["laplace"]["radius"] / result["f64"]["laplace"]["radius"]
                                   - 1.0)
+        dmax, dmean, nif, flips = interface_displacement(L.colour(), fields["colour"],
+                                                         np.ones((n, n, n), bool))
+        res.update({"interface_disp_max": dmax, "interface_disp_mean": dmean,
+                    "interface_sites": nif, "colour_sign_flips": flips})
         result[st] = res
         del L
         torch.cuda.empty_cache()

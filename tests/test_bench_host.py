@@ -72,3 +72,28 @@ def test_ensemble_parameters_in_range_and_seeded():
         g = p["gcc"][0][1]
         assert 3.0 <= g <= 4.5 and p["gcc"][1][0] == g
         assert 0.2 <= p["initRadius"] <= 0.35
+
+
+def test_interface_displacement_of_a_shifted_profile():
+    """A colour profile cos(k (x - a)) shifted by delta: the first-order
+    displacement at the interface sites is delta k / sin(k) (the central
+    difference's factor), the two crossings give 2 x 2 x 64 interface sites,
+    and a shift by one site turns one plane per crossing to the other sign."""
+    n = 64
+    x = np.arange(n, dtype=float)[:, None, None] * np.ones((1, 8, 8))
+    k = 2 * np.pi / n
+    ref = np.cos(k * (x - 0.3))
+    fluid = np.ones(ref.shape, bool)
+    delta = 1e-3
+    dmax, dmean, nif, flips = bench.interface_displacement(np.cos(k * (x - 0.3 - delta)), ref,
+                                                           fluid)
+    assert nif == 2 * 2 * 64 and flips == 0
+    want = delta * k / np.sin(k)
+    assert abs(dmax - want) < 0.01 * want and abs(dmean - want) < 0.01 * want
+    *_, flips = bench.interface_displacement(np.cos(k * (x - 1.3)), ref, fluid)
+    assert flips == 2 * 64
+    # a solid plane at x = 17 hides the first crossing (16.3): only the
+    # second one's sites count
+    fluid[17] = False
+    dmax, _, nif, flips = bench.interface_displacement(ref, ref, fluid)
+    assert nif == 2 * 64 and dmax == 0.0 and flips == 0
