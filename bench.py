@@ -460,8 +460,9 @@ def ensemble_params(count=256, seed=20261202):
 
 
 def ensemble_workload(D, count, steps, storage="f32_shift"):
-    """The ensemble advanced by one persistent launch per step (graph replay,
-    autotuned tiles) and by one launch per lattice and step; MLUPS counts
+    """The ensemble advanced by the autotuned batch kernel (graph replay
+    where it applies), by one launch per lattice and step, and by the split
+    pair as two launches and as one persistent launch per step; MLUPS counts
     sum(L^3) per step."""
     import torch
     params = ensemble_params(count)
@@ -486,10 +487,19 @@ def ensemble_workload(D, count, steps, storage="f32_shift"):
     dt_graph = timed(lambda: B.step(steps, graph=True))
     dt_eager = timed(lambda: B.step(steps, graph=False))
     dt_loop = timed(lambda: B.step_looped(steps))
+    # the split pair in both of its forms, whatever the autotuner picked: two
+    # launches per step (graph replay) and one persistent cooperative launch
+    forms = {}
+    for kn, name in ((2, "split_pair_graph_mlups"), (3, "split_persistent_mlups")):
+        B.set_config(cfg[0], cfg[1], cfg[2], kn)
+        B.step(2)
+        forms[name] = round(sites * steps / timed(lambda: B.step(steps, graph=True)) / 1e6, 1)
+    B.set_config(*cfg)
     finite = all(np.isfinite(L.total_mass(0)) for L in lats)
     res = {"lattices": count, "sites": sites, "steps": steps, "storage": storage,
            "config": {"tile_rows": cfg[0], "chunk": cfg[1], "workgroups": cfg[2],
-                      "kernel": ["tiled, solid-capable", "tiled, no-solid", "split"][cfg[3]]},
+                      "kernel": ["tiled, solid-capable", "tiled, no-solid", "split pair",
+                                 "split pair, one persistent launch"][cfg[3]]},
            "autotune_launch_ms": round(ms, 4),
            "batched_graph_mlups": round(sites * steps / dt_graph / 1e6, 1),
            # whole-step algorithmic bytes (populations read and written once per
@@ -498,7 +508,8 @@ def ensemble_workload(D, count, steps, storage="f32_shift"):
                                             / HBM_PEAK, 4),
            "batched_eager_mlups": round(sites * steps / dt_eager / 1e6, 1),
            "looped_mlups": round(sites * steps / dt_loop / 1e6, 1),
-           "speedup_vs_looped": round(dt_loop / dt_graph, 3), "finite": bool(finite)}
+           "speedup_vs_looped": round(dt_loop / dt_graph, 3), **forms,
+           "finite": bool(finite)}
     print(json.dumps({"workload": "ensemble of %d D3Q19 two-component droplet lattices, "
                                   "L ~ U{32..96}" % count, "results": res}), flush=True)
 

@@ -1710,7 +1710,10 @@ struct dlbc_batch {
   std::vector<BatchItem> h_items;
   std::vector<int> lat_first;  // first item of each lattice, and the total
   int ty = 8, chunk = 1 << 20, grid = 512;
-  int kernel = 0;  // tiled: 0 solid-capable, 1 without solids; 2: the split pair
+  int kernel = 0;  // tiled: 0 solid-capable, 1 without solids; 2: the split pair;
+                   // 3: the split pair as one persistent (cooperative) launch
+  int coop_grid = 0;     // workgroups of that launch (0: not yet queried)
+  int coop_variant = -1;  // the kernel variant coop_grid was queried for
   int *d_first = nullptr;     // split: first 128-site block of each member, and the total
   std::vector<int> h_first;
   bool items_dirty = true;
@@ -1824,20 +1827,73 @@ int batch_prepare(dlbc_batch *b) {
   return DLBC_OK;
 }
 
+// Whether every member's element offsets fit the split kernels' 32-bit lane
+// offsets (the OFF32 variants).
+bool batch_off32(const dlbc_batch *b) {
+  for (auto *c : b->ctx)
+    if (!split_off32(c->geo)) return false;
+  return true;
+}
+
 // Split schedule over the members [l0, l1).
 int batch_launch_split(dlbc_batch *b, int l0, int l1, int parity) {
   const int nlat = (int)b->ctx.size();
   const int blk0 = b->h_first[l0], nblk = b->h_first[l1] - blk0;
   const bool psi = batch_has_sc(b, l0, l1);
-  if (b->storage == DLBC_STORAGE_F32_SHIFT)
-    launch_batch_split<float>(b->d_lats, b->d_first, nlat, blk0, nblk, parity, psi, b->stream);
+  const bool f32 = b->storage == DLBC_STORAGE_F32_SHIFT, o32 = batch_off32(b);
+  if (f32 && o32)
+    launch_batch_split<float, true>(b->d_lats, b->d_first, nlat, blk0, nblk, parity, psi,
+                                    b->stream);
+  else if (f32)
+    launch_batch_split<float, false>(b->d_lats, b->d_first, nlat, blk0, nblk, parity, psi,
+                                     b->stream);
+  else if (o32)
+    launch_batch_split<__half, true>(b->d_lats, b->d_first, nlat, blk0, nblk, parity, psi,
+                                     b->stream);
   else
-    launch_batch_split<__half>(b->d_lats, b->d_first, nlat, blk0, nblk, parity, psi, b->stream);
+    launch_batch_split<__half, false>(b->d_lats, b->d_first, nlat, blk0, nblk, parity, psi,
+                                      b->stream);
   BHIPCHK(b, hipGetLastError());
   return DLBC_OK;
 }
 
+// The split pair over every member as one persistent launch.
+int batch_launch_persistent(dlbc_batch *b, int parity) {
+  const int nlat = (int)b->ctx.size(), nblk = b->h_first[nlat];
+  if (nblk <= 0) return DLBC_OK;
+  const bool psi = batch_has_sc(b, 0, nlat);
+  const bool f32 = b->storage == DLBC_STORAGE_F32_SHIFT, o32 = batch_off32(b);
+  // the kernel (and so its occupancy) follows the storage and the offset width
+  const int variant = (f32 ? 2 : 0) | (o32 ? 1 : 0);
+  if (b->coop_grid <= 0 || b->coop_variant != variant) {
+    b->coop_grid = f32 ? (o32 ? batch_split_persistent_grid<float, true>(b->device, nblk)
+                              : batch_split_persistent_grid<float, false>(b->device, nblk))
+                       : (o32 ? batch_split_persistent_grid<__half, true>(b->device, nblk)
+                              : batch_split_persistent_grid<__half, false>(b->device, nblk));
+    if (b->coop_grid <= 0)
+      return bfail(b, DLBC_ERR_HIP, "persistent split: occupancy query failed");
+    b->coop_variant = variant;
+  }
+  const int grid = std::min(b->coop_grid, nblk);
+  hipError_t e;
+  if (f32 && o32)
+    e = launch_batch_split_persistent<float, true>(b->d_lats, b->d_first, nlat, nblk, parity,
+                                                   psi, grid, b->stream);
+  else if (f32)
+    e = launch_batch_split_persistent<float, false>(b->d_lats, b->d_first, nlat, nblk, parity,
+                                                    psi, grid, b->stream);
+  else if (o32)
+    e = launch_batch_split_persistent<__half, true>(b->d_lats, b->d_first, nlat, nblk, parity,
+                                                    psi, grid, b->stream);
+  else
+    e = launch_batch_split_persistent<__half, false>(b->d_lats, b->d_first, nlat, nblk, parity,
+                                                     psi, grid, b->stream);
+  BHIPCHK(b, e);
+  return DLBC_OK;
+}
+
 int batch_launch(dlbc_batch *b, const BatchItem *items, int nitems, int parity, int grid) {
+  if (b->kernel == 3) return batch_launch_persistent(b, parity);
   if (b->kernel == 2) return batch_launch_split(b, 0, (int)b->ctx.size(), parity);
   const bool solid_kernel = b->kernel == 0 || batch_has_solid(b);
   const int rc = b->storage == DLBC_STORAGE_F32_SHIFT
@@ -1906,7 +1962,7 @@ int dlbc_batch_create(dlbc_batch **out, dlbc_gpu_ctx *const *lattices, int n) {
 
 int dlbc_batch_set_config(dlbc_batch *b, int tile_y, int chunk, int workgroups, int kernel) {
   if (!b || (tile_y != 4 && tile_y != 8) || chunk < 4 || workgroups < 8 || kernel < 0 ||
-      kernel > 2)
+      kernel > 3)
     return DLBC_ERR_ARG;
   if (tile_y != b->ty || chunk != b->chunk) b->items_dirty = true;
   if (workgroups != b->grid || kernel != b->kernel) batch_drop_graph(b);
@@ -1933,7 +1989,9 @@ int dlbc_batch_step(dlbc_batch *b, int nsteps, int use_graph) {
   if (rc) return rc;
   const int nit = (int)b->h_items.size();
   int t = 0;
-  if (use_graph && nsteps >= 2) {
+  // the persistent split launch is cooperative and is not captured: one
+  // launch per step either way
+  if (use_graph && nsteps >= 2 && b->kernel != 3) {
     if (!b->graph) {  // two steps: even parity, then odd
       hipGraph_t g = nullptr;
       BHIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -1967,7 +2025,7 @@ int dlbc_batch_step_looped(dlbc_batch *b, int nsteps) {
   for (int t = 0; t < nsteps; ++t)
     for (size_t l = 0; l + 1 < b->lat_first.size(); ++l) {
       const int first = b->lat_first[l], cnt = b->lat_first[l + 1] - first;
-      rc = b->kernel == 2 ? batch_launch_split(b, (int)l, (int)l + 1, t & 1)
+      rc = b->kernel >= 2 ? batch_launch_split(b, (int)l, (int)l + 1, t & 1)
                           : batch_launch(b, b->d_items + first, cnt, t & 1,
                                          std::min(b->grid, (cnt + 7) / 8 * 8));
       if (rc) return rc;
@@ -2001,14 +2059,14 @@ int dlbc_batch_autotune(dlbc_batch *b, int *out, double *best_ms) {
     ms_out = ms / 3.0;
     return DLBC_OK;
   };
-  {
-    b->kernel = 2;  // the split pair: no tile or chunk to tune
+  for (int kn : {2, 3}) {  // the split pair, two launches or one: no tile or chunk to tune
+    b->kernel = kn;
     double ms;
     rc = trial(0, b->grid, ms);
     if (rc) return rc;
     if (ms < best) {
       best = ms;
-      bk = 2;
+      bk = kn;
     }
   }
   for (int ty : tys)

@@ -258,15 +258,17 @@ int dlbc_batch_create(dlbc_batch **out, dlbc_gpu_ctx *const *lattices, int n);
  * (solid-capable, 128 VGPRs), 1 the tiled kernel for batches without solid
  * sites (more loads in flight), 2 the split pair (a psi launch and a
  * stream/force/collide launch per step over all members; tile, chunk and
- * workgroups do not apply).  Defaults 8, whole lattice, 512, 0. */
+ * workgroups do not apply), 3 the split pair as one persistent cooperative
+ * launch per step (a grid barrier between the passes; every workgroup
+ * resident; not captured into graphs).  Defaults 8, whole lattice, 512, 0. */
 int dlbc_batch_set_config(dlbc_batch *b, int tile_y, int chunk, int workgroups, int kernel);
 int dlbc_batch_get_config(const dlbc_batch *b, int *out /*4*/);
 /* Time every (tile height, chunk, workgroups, kernel) candidate on the
  * current state (reads it, writes only the buffer the next step overwrites)
  * and keep the fastest; out receives it, best_ms its launch time. */
 int dlbc_batch_autotune(dlbc_batch *b, int *out /*4*/, double *best_ms);
-/* nsteps steps, one launch per step; with use_graph pairs of steps are
- * replayed from a captured HIP graph. */
+/* nsteps steps, one launch per step (two for kernel 2); with use_graph
+ * pairs of steps are replayed from a captured HIP graph (not for kernel 3). */
 int dlbc_batch_step(dlbc_batch *b, int nsteps, int use_graph);
 /* The same steps as one launch per member and step (the per-lattice loop). */
 int dlbc_batch_step_looped(dlbc_batch *b, int nsteps);

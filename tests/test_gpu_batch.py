@@ -70,7 +70,7 @@ def test_batch_equals_lattices_stepped_alone(D, storage, kernel):
     _same(_pops(lats2), ref, "looped")
 
 
-@pytest.mark.parametrize("kernel", [0, 2], ids=["tiled", "split"])
+@pytest.mark.parametrize("kernel", [0, 2, 3], ids=["tiled", "split", "split-persistent"])
 def test_batch_graph_replays(D, kernel):
     """The captured two-step graph replayed several times, with eager work
     (field downloads, an eager step, set_params) between the replays: every
@@ -100,7 +100,7 @@ def test_batch_graph_replays(D, kernel):
 def test_batch_config_and_autotune_do_not_change_results(D):
     ref = None
     for cfg in [(8, 1 << 20, 512, 0), (4, 16, 64, 1), (8, 8, 2048, 1), (4, 32, 8, 0),
-                (8, 16, 256, 2)]:
+                (8, 16, 256, 2), (8, 16, 256, 3)]:
         lats = _members(D)
         B = D.Batch(lats)
         B.set_config(*cfg)
@@ -262,7 +262,7 @@ def test_config5_ensemble_16_lattices(D):
     assert worst["drho"] <= 3e-6 and worst["du"] <= 3e-6, worst
 
 
-@pytest.mark.parametrize("kernel", [1, 2], ids=["tiled-no-solid", "split"])
+@pytest.mark.parametrize("kernel", [1, 2, 3], ids=["tiled-no-solid", "split", "split-persistent"])
 def test_batch_graph_after_member_gains_solids(D, kernel):
     """The captured batch graph is dropped when a member gains solid sites (the
     tiled kernel without bounce-back was captured) or when Shan-Chen is turned
