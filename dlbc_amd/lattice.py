@@ -483,6 +483,8 @@ class Batch:
             pass
 
     def set_config(self, tile_y, chunk, workgroups, kernel=0):
+        """kernel: 0 / 1 the tiled persistent kernel (with / without solids),
+        2 the split pair, 3 the split pair as one persistent launch."""
         self._check(lib().dlbc_batch_set_config(self.h, tile_y, chunk, workgroups, kernel),
                     "set_config")
 
